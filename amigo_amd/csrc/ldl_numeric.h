@@ -1,0 +1,108 @@
+// Numeric multifrontal LDL^T on the GPU (host orchestration).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "symbolic.h"
+
+namespace amigo_amd {
+
+// Device-side copy of the symbolic plan plus derived maps.
+struct DevPlan {
+  int n = 0, K = 0;
+  const int* ns = nullptr;      // [K] pivots of the supernode
+  const int* cs = nullptr;      // [K] static contribution rows
+  const int* cptr = nullptr;    // [K+1]
+  const int* crows = nullptr;   // contribution rows (original numbering)
+  const int* cmap = nullptr;    // static position of each row in the parent
+  const int* kptr = nullptr;    // [K+2] children pointer
+  const int* kids = nullptr;    // [K]
+  const int* nptr = nullptr;    // [K+1]
+  const int* nvars = nullptr;   // [n] node -> variables (original)
+  const int* parent = nullptr;  // [K]
+  const int* aptr = nullptr;    // [K+1] original-entry assembly map
+  const int* aidx = nullptr;    // CSR value index
+  const int* arow = nullptr;    // static row position in the front
+  const int* acol = nullptr;    // column (pivot index < ns)
+};
+
+// Dynamic factor state.
+struct DevState {
+  int* D = nullptr;        // [K] delayed pivots received from the children
+  int* npiv = nullptr;     // [K]
+  int* ndel = nullptr;     // [K] delayed pivots passed to the parent
+  int* m = nullptr;        // [K] front size
+  int64_t* foff = nullptr; // [K] offset of the front in rpool (doubles)
+  int64_t* voff = nullptr; // [K] offset of the front variables in ipool
+  int* pstate = nullptr;   // [K*4] blocked-path state: k, nc, done, unused
+  double* rpool = nullptr;
+  int* ipool = nullptr;
+  unsigned long long* tops = nullptr;  // [2] bump pointers (rpool, ipool)
+  unsigned long long rcap = 0, icap = 0;
+  int* flags = nullptr;    // [8]: 0 error, 1..3 inertia pos/neg/zero
+};
+
+struct FactorParams {
+  double ustab;
+  double pivot_tol;
+};
+
+class LdlNumeric {
+ public:
+  LdlNumeric() = default;
+  ~LdlNumeric();
+  // Upload the plan; returns 0 or an error code (message in err).
+  int setup(const SymbolicPlan& plan, const int* rowp, const int* cols,
+            double delay_growth, std::string& err);
+  int factor(const double* vals, const double* shift, double ustab,
+             double pivot_tol, bool cholesky, hipStream_t stream,
+             std::string& err);
+  int solve(int nrhs, double* x, int ldx, hipStream_t stream,
+            std::string& err);
+  void inertia(int* pos, int* neg, int* zero) const {
+    *pos = inertia_[0];
+    *neg = inertia_[1];
+    *zero = inertia_[2];
+  }
+  int pivot_counts(int* np, int* nd, std::string& err) const;
+  double factor_flops() const { return flops_; }
+  double solve_bytes() const { return solve_bytes_; }
+  int num_levels() const { return int(level_ptr_.size()) - 1; }
+  int max_front() const { return max_front_; }
+  bool factored() const { return factored_; }
+
+ private:
+  int run_factor(const double* vals, const double* shift, FactorParams prm,
+                 bool cholesky, hipStream_t stream, std::string& err);
+  int ensure_pools(unsigned long long rcap, unsigned long long icap,
+                   std::string& err);
+
+  int n_ = 0, K_ = 0;
+  DevPlan P_;
+  DevState S_;
+  std::vector<void*> allocs_;
+  // Levels: nodes grouped by height in the node tree (leaves = 0).
+  std::vector<int> level_ptr_, level_nodes_;
+  std::vector<int> level_is_big_;   // per level: has big fronts
+  std::vector<int> big_nodes_;      // per level list of big fronts
+  std::vector<int> big_ptr_;
+  std::vector<int> level_lds_;      // LDS doubles per level kernel
+  int* d_level_nodes_ = nullptr;
+  int* d_big_mask_ = nullptr;       // [K] 1 if front is factored blocked
+  std::vector<int> h_ns_, h_cs_;
+  int max_front_ = 0;
+  double* W_ = nullptr;             // panel workspace (max_m x 64)
+  size_t W_cap_ = 0;
+  double delay_growth_ = 2.0;
+  unsigned long long base_rcap_ = 0, base_icap_ = 0;
+  int inertia_[3] = {0, 0, 0};
+  double flops_ = 0.0, solve_bytes_ = 0.0;
+  bool factored_ = false;
+  int max_p_ = 0;                   // largest fully summed block seen
+};
+
+}  // namespace amigo_amd

@@ -1,0 +1,130 @@
+/*
+ * amigo_amd — MI355X-native backend for Amigo's interior-point KKT path.
+ *
+ * C ABI (plain pointers and sizes; device pointers are HIP device memory,
+ * `stream` is a hipStream_t passed as void*, NULL = default stream).
+ * Every entry point returns 0 on success or a negative AMIGO_ERR_* code;
+ * amigo_last_error() gives a message.  No entry point aborts.
+ *
+ * Each declaration names the reference interface it replaces.
+ */
+#ifndef AMIGO_AMD_H
+#define AMIGO_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  AMIGO_OK = 0,
+  AMIGO_ERR_ARG = -1,        /* invalid argument / shape */
+  AMIGO_ERR_HIP = -2,        /* HIP runtime error */
+  AMIGO_ERR_STATE = -3,      /* call out of order (e.g. solve before factor) */
+  AMIGO_ERR_CAPACITY = -4,   /* delayed pivots exceeded the front pool */
+  AMIGO_ERR_NOMEM = -5,      /* device allocation failed */
+};
+
+/* Orderings, numbered as include/ordering_utils.h:21 OrderingType. */
+enum {
+  AMIGO_ORDER_NESTED_DISSECTION = 0,
+  AMIGO_ORDER_AMD = 1,
+  AMIGO_ORDER_MULTI_COLOR = 2,
+  AMIGO_ORDER_NATURAL = 3,
+};
+
+/* Solver types, as include/sparse_ldl.h:23 SparseLDL::SolverType. */
+enum { AMIGO_SOLVER_LDL = 0, AMIGO_SOLVER_CHOLESKY = 1 };
+
+/* Factor precisions (this build's mixed-precision form). */
+enum { AMIGO_PREC_FP64 = 0, AMIGO_PREC_MIXED = 1 };
+
+const char* amigo_last_error(void);
+
+/* ---------------- ordering / symbolic analysis (host) ---------------- */
+
+/* OrderingUtils::copy_for_reorder + OrderingUtils::reorder
+ * (include/ordering_utils.h:313-360, :38-56): perm[new] = old,
+ * iperm[old] = new.  AMD is bit-identical to BlockAMD::amd
+ * (include/block_amd.h:27-452); ND is this build's own recursive bisection. */
+int amigo_order(int order, int n, const int* rowp, const int* cols, int* perm,
+                int* iperm);
+
+/* BlockAMD::amd on a prepared pattern (diagonal included, rows sorted) with
+ * multipliers ordered last (include/block_amd.h:27, :77). */
+int amigo_block_amd(int n, const int* rowp, const int* cols, int nmult,
+                    const int* mult, int* perm);
+
+/* Etree / postorder / column counts in the permuted order
+ * (include/sparse_ldl.h:2375-2701).  perm/iperm may be NULL. */
+int amigo_etree_post_colcount(int n, const int* rowp, const int* cols,
+                              const int* perm, const int* iperm, int* parent,
+                              int* post, int* colcount);
+
+/* ---------------- SparseLDL (symbolic on host, numeric on GPU) ---------- */
+
+typedef struct amigo_ldl amigo_ldl;
+
+/* SparseLDL<T>::SparseLDL(mat, solver_type, ustab, pivot_tol, delay_growth,
+ * order) (include/sparse_ldl.h:65-118).  Runs the symbolic analysis on the
+ * full symmetric CSR pattern (sorted rows) and uploads the front plan. */
+int amigo_ldl_create(int n, const int* rowp, const int* cols, int solver_type,
+                     double ustab, double pivot_tol, double delay_growth,
+                     int order, amigo_ldl** out);
+int amigo_ldl_destroy(amigo_ldl* h);
+
+/* Sizes of the symbolic arrays: num_nodes, total contribution rows, number
+ * of elimination-tree levels, and the largest static front. */
+int amigo_ldl_sizes(const amigo_ldl* h, int* num_nodes, int* contrib_total,
+                    int* num_levels, int* max_front);
+
+/* Copies of the symbolic arrays in the reference's layout
+ * (sparse_ldl.h:3252-3268): node_ptr[K+1], node_to_var[n],
+ * children_ptr[K+2], children[K], contrib_ptr[K+1],
+ * contrib_rows[contrib_total], invperm[n] (identity for NATURAL). */
+int amigo_ldl_get_symbolic(const amigo_ldl* h, int* node_ptr, int* node_to_var,
+                           int* children_ptr, int* children, int* contrib_ptr,
+                           int* contrib_rows, int* invperm);
+
+/* SparseLDL::factor() (include/sparse_ldl.h:136-151): numeric multifrontal
+ * LDL^T of the CSR values (device pointer, nnz doubles, same pattern as at
+ * create).  precision = AMIGO_PREC_FP64 or AMIGO_PREC_MIXED.  Synchronous:
+ * returns after the inertia is known. */
+int amigo_ldl_factor(amigo_ldl* h, const double* values_dev, int precision,
+                     void* stream);
+
+/* Same as amigo_ldl_factor with the diagonal shift values[diag[i]] +=
+ * shift_dev[i] fused into the front assembly (CSRMat::add_diagonal,
+ * include/csr_matrix.h:508-528, followed by factor()). shift_dev may be
+ * NULL. */
+int amigo_ldl_factor_shifted(amigo_ldl* h, const double* values_dev,
+                             const double* shift_dev, int precision,
+                             void* stream);
+
+/* SparseLDL::get_inertia (include/sparse_ldl.h:198-206).  npos counts zero
+ * pivots as positive like the reference; nzero reports them separately. */
+int amigo_ldl_get_inertia(const amigo_ldl* h, int* npos, int* nneg,
+                          int* nzero);
+
+/* Per-node pivot and delayed-pivot counts of the last factorization
+ * (MatrixFactor::get_factor, include/sparse_ldl.h:411-445). */
+int amigo_ldl_pivot_counts(const amigo_ldl* h, int* num_pivots,
+                           int* num_delayed);
+
+/* SparseLDL::solve(nrhs, x, ldx) (include/sparse_ldl.h:184-190): x is a
+ * device column-major n x nrhs block, overwritten in place. */
+int amigo_ldl_solve(amigo_ldl* h, int nrhs, double* x_dev, int ldx,
+                    void* stream);
+
+/* Flop count of the last factorization per the algorithmic definition
+ * sum over fronts sum_k [(m-k-1) + (m-k-1)(m-k)], and bytes streamed by
+ * one solve with one right-hand side. */
+int amigo_ldl_work(const amigo_ldl* h, double* factor_flops,
+                   double* solve_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AMIGO_AMD_H */
