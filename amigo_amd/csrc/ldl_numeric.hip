@@ -302,12 +302,15 @@ __global__ __launch_bounds__(NT) void level_kernel(
   __shared__ int sh[4];
   __shared__ int cnt[3];
   const int tid = threadIdx.x;
-  if (S.flags[0] != 0) return;
   const int node = lnodes[blockIdx.x];
   const int ns = P.ns[node], cs = P.cs[node];
   const int kb = P.kptr[node], ke = P.kptr[node + 1];
 
   if (tid == 0) {
+    sh[0] = -1;
+    cnt[0] = cnt[1] = cnt[2] = 0;
+  }
+  if (tid == 0 && atomicAdd(&S.flags[0], 0) == 0) {
     int D = 0;
     for (int q = kb; q < ke; q++) D += S.ndel[P.kids[q]];
     const int m = ns + D + cs;
@@ -325,7 +328,6 @@ __global__ __launch_bounds__(NT) void level_kernel(
       S.D[node] = D;
       S.m[node] = m;
     }
-    cnt[0] = cnt[1] = cnt[2] = 0;
   }
   __syncthreads();
   if (sh[0] < 0) return;
@@ -432,8 +434,11 @@ __global__ __launch_bounds__(NT) void panel_kernel(DevState S, int node,
                                                    int chol) {
   __shared__ VI red[NT / 64];
   __shared__ int cnt[3];
+  __shared__ int bad;
   const int tid = threadIdx.x;
-  if (S.flags[0] != 0) return;
+  if (tid == 0) bad = atomicAdd(&S.flags[0], 0);
+  __syncthreads();
+  if (bad != 0) return;
   int* ps = S.pstate + 4 * node;
   const int m = S.m[node];
   double* F = S.rpool + S.foff[node];
@@ -600,7 +605,10 @@ __global__ __launch_bounds__(NT) void trailing_kernel(DevState S, int node,
                                                       const double* __restrict__ W) {
   __shared__ double As[TILE][NB + 1];
   __shared__ double Bs[TILE][NB + 1];
-  if (S.flags[0] != 0) return;
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = atomicAdd(&S.flags[0], 0);
+  __syncthreads();
+  if (bad != 0) return;
   const int* ps = S.pstate + 4 * node;
   const int m = S.m[node];
   const int k = ps[0], kstart = ps[2], kw = ps[3];

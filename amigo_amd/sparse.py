@@ -1,0 +1,246 @@
+"""Device-resident CSR matrix and the SparseLDL / SparseCholesky drop-ins.
+
+Mirrors the reference's Python surface (amigo/amigo.cpp:604-694):
+
+    ldl = SparseLDL(mat, solver_type="LDL", ustab=0.01, pivot_tol=1e-14,
+                    delay_growth=2.0, order="NATURAL")
+    ldl.factor()            # -> 0
+    ldl.solve(x)            # in place; x is a device tensor (n,) or (n, nrhs)
+    ldl.get_inertia()       # -> (npos, nneg)
+
+The symbolic analysis runs on the host in the constructor (as in
+include/sparse_ldl.h:65-118); values and factors live in HBM.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+LDL = "LDL"
+CHOLESKY = "CHOLESKY"
+
+_ORDER = _lib.AMIGO_ORDER
+
+
+def _order_code(order):
+    if isinstance(order, int):
+        return order
+    key = str(order).upper().split(".")[-1]
+    if key == "DEFAULT":
+        key = "AMD"
+    return _ORDER[key]
+
+
+def current_stream_ptr():
+    return torch.cuda.current_stream().cuda_stream
+
+
+class CSRMat:
+    """CSR matrix with the full symmetric pattern (sorted rows).
+
+    The pattern is kept on the host (int32) and, once ``to_device`` is
+    called, also on the device; values live in a float64 device tensor.
+    """
+
+    def __init__(self, nrows, ncols, rowp, cols, data=None, device="cuda"):
+        self.nrows = int(nrows)
+        self.ncols = int(ncols)
+        rowp = np.ascontiguousarray(rowp, dtype=np.int64)
+        cols = np.ascontiguousarray(cols, dtype=np.int64)
+        if rowp.shape != (self.nrows + 1,) or rowp[0] != 0:
+            raise ValueError("rowp must have nrows + 1 entries starting at 0")
+        # Sort each row (values follow) so that the pattern meets the
+        # solver's requirement.
+        order = np.lexsort((cols, np.repeat(np.arange(self.nrows), np.diff(rowp))))
+        self.rowp = rowp.astype(np.int32)
+        self.cols = cols[order].astype(np.int32)
+        self.nnz = int(self.rowp[-1])
+        self.device = torch.device(device)
+        if data is None:
+            vals = np.zeros(self.nnz)
+        else:
+            vals = np.asarray(data, dtype=np.float64)[order]
+        self.data = torch.as_tensor(vals, dtype=torch.float64, device=self.device)
+        self._diag = None
+        self._rowp_d = None
+        self._cols_d = None
+
+    @classmethod
+    def from_scipy(cls, A, device="cuda"):
+        A = A.tocsr()
+        return cls(A.shape[0], A.shape[1], A.indptr, A.indices, A.data, device=device)
+
+    def diag_indices(self):
+        """CSR position of each diagonal entry (-1 when absent)."""
+        if self._diag is None:
+            d = np.full(self.nrows, -1, dtype=np.int64)
+            for i in range(self.nrows):
+                s, e = self.rowp[i], self.rowp[i + 1]
+                k = s + np.searchsorted(self.cols[s:e], i)
+                if k < e and self.cols[k] == i:
+                    d[i] = k
+            self._diag = d
+        return self._diag
+
+    def pattern_device(self):
+        if self._rowp_d is None:
+            self._rowp_d = torch.as_tensor(self.rowp, device=self.device)
+            self._cols_d = torch.as_tensor(self.cols, device=self.device)
+        return self._rowp_d, self._cols_d
+
+    def get_data(self):
+        return self.data.cpu().numpy()
+
+    def set_data(self, values):
+        self.data.copy_(torch.as_tensor(np.asarray(values, dtype=np.float64)))
+
+    def to_scipy(self):
+        from scipy.sparse import csr_matrix
+
+        return csr_matrix(
+            (self.get_data(), self.cols, self.rowp), shape=(self.nrows, self.ncols)
+        )
+
+
+class SparseLDL:
+    """Multifrontal LDL^T with symbolic analysis on the host and numeric
+    factorization / solves on the GPU (include/sparse_ldl.h:19-206)."""
+
+    def __init__(
+        self,
+        mat,
+        solver_type=LDL,
+        ustab=0.01,
+        pivot_tol=1e-14,
+        delay_growth=2.0,
+        order="NATURAL",
+        precision="fp64",
+    ):
+        self.mat = mat
+        self.n = mat.nrows
+        self.solver_type = CHOLESKY if str(solver_type).upper().endswith("CHOLESKY") else LDL
+        self.precision = precision
+        h = ctypes.c_void_p()
+        _lib.check(
+            _lib.lib().amigo_ldl_create(
+                self.n,
+                _lib.ip(mat.rowp),
+                _lib.ip(mat.cols),
+                1 if self.solver_type == CHOLESKY else 0,
+                float(ustab),
+                float(pivot_tol),
+                float(delay_growth),
+                _order_code(order),
+                ctypes.byref(h),
+            )
+        )
+        self._h = h
+        self._factored = False
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().amigo_ldl_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # ---- symbolic information -------------------------------------------
+    def sizes(self):
+        K, C, L, M = (ctypes.c_int() for _ in range(4))
+        _lib.check(
+            _lib.lib().amigo_ldl_sizes(
+                self._h, ctypes.byref(K), ctypes.byref(C), ctypes.byref(L), ctypes.byref(M)
+            )
+        )
+        return {"num_nodes": K.value, "contrib_total": C.value, "num_levels": L.value, "max_front": M.value}
+
+    def get_symbolic(self):
+        s = self.sizes()
+        K, C, n = s["num_nodes"], s["contrib_total"], self.n
+        arrs = [np.zeros(k, np.int32) for k in (K + 1, n, K + 2, K, K + 1, C, n)]
+        _lib.check(_lib.lib().amigo_ldl_get_symbolic(self._h, *[_lib.ip(a) for a in arrs]))
+        names = ["node_ptr", "node_to_var", "children_ptr", "children", "contrib_ptr", "contrib_rows", "invperm"]
+        return dict(zip(names, arrs))
+
+    # ---- numeric --------------------------------------------------------
+    def factor(self, shift=None, precision=None):
+        """Numeric factorization of mat.data (+ optional diagonal shift)."""
+        prec = precision or self.precision
+        code = 1 if prec == "mixed" else 0
+        vals = self.mat.data
+        if shift is None:
+            rc = _lib.lib().amigo_ldl_factor(
+                self._h, _lib.vp(vals.data_ptr()), code, _lib.vp(current_stream_ptr())
+            )
+        else:
+            assert shift.dtype == torch.float64 and shift.is_cuda and shift.numel() == self.n
+            rc = _lib.lib().amigo_ldl_factor_shifted(
+                self._h,
+                _lib.vp(vals.data_ptr()),
+                _lib.vp(shift.data_ptr()),
+                code,
+                _lib.vp(current_stream_ptr()),
+            )
+        _lib.check(rc)
+        self._factored = True
+        return 0
+
+    def get_inertia(self):
+        p, q, z = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(
+            _lib.lib().amigo_ldl_get_inertia(self._h, ctypes.byref(p), ctypes.byref(q), ctypes.byref(z))
+        )
+        return p.value, q.value
+
+    def get_inertia3(self):
+        p, q, z = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(
+            _lib.lib().amigo_ldl_get_inertia(self._h, ctypes.byref(p), ctypes.byref(q), ctypes.byref(z))
+        )
+        return p.value - z.value, q.value, z.value
+
+    def pivot_counts(self):
+        K = self.sizes()["num_nodes"]
+        a = np.zeros(K, np.int32)
+        b = np.zeros(K, np.int32)
+        _lib.check(_lib.lib().amigo_ldl_pivot_counts(self._h, _lib.ip(a), _lib.ip(b)))
+        return a, b
+
+    def solve(self, x):
+        """Solve in place.  x: float64 device tensor (n,) or column-major
+        (n, nrhs) (i.e. x.t() contiguous)."""
+        if not (x.is_cuda and x.dtype == torch.float64):
+            raise TypeError("solve expects a float64 device tensor")
+        if x.dim() == 1:
+            if not x.is_contiguous():
+                raise ValueError("x must be contiguous")
+            nrhs, ldx = 1, self.n
+        else:
+            if x.shape[0] != self.n or x.stride(0) != 1:
+                raise ValueError("2-D x must be column-major (n, nrhs)")
+            nrhs, ldx = x.shape[1], x.stride(1)
+        _lib.check(
+            _lib.lib().amigo_ldl_solve(
+                self._h, nrhs, _lib.vp(x.data_ptr()), ldx, _lib.vp(current_stream_ptr())
+            )
+        )
+        return x
+
+    def work(self):
+        f, b = ctypes.c_double(), ctypes.c_double()
+        _lib.check(_lib.lib().amigo_ldl_work(self._h, ctypes.byref(f), ctypes.byref(b)))
+        return f.value, b.value
+
+
+class SparseCholesky(SparseLDL):
+    """Drop-in for SparseCholesky<T> (include/sparse_cholesky.h:64-376):
+    a factorization without pivoting of a symmetric (quasi-)definite matrix.
+    The reference orders naturally; so does this one."""
+
+    def __init__(self, mat):
+        super().__init__(mat, solver_type=CHOLESKY, order="NATURAL")

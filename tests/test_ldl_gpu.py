@@ -1,0 +1,141 @@
+"""GPU parity of the multifrontal LDL^T against the reference's SparseLDL.
+
+For every matrix: the inertia equals the reference's (and the eigenvalue
+inertia for the small ones), and the solve residual ||Ax - b|| / ||b|| is at
+FP64 level.  Per-node pivot/delay counts are compared with the reference.
+"""
+
+import numpy as np
+import pytest
+
+from tests import matrices as M
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _ref():
+    from oracle import ref
+
+    if not ref.available():
+        pytest.skip("reference oracle library not built")
+    return ref
+
+
+def _gpu_ldl(A, order, ustab=0.01, delay_growth=2.0, precision="fp64"):
+    from amigo_amd.sparse import CSRMat, SparseLDL
+
+    mat = CSRMat.from_scipy(A)
+    ldl = SparseLDL(mat, ustab=ustab, delay_growth=delay_growth, order=order, precision=precision)
+    assert ldl.factor() == 0
+    return mat, ldl
+
+
+def _residual(A, ldl, nrhs=1, seed=0):
+    rng = np.random.default_rng(seed)
+    n = A.shape[0]
+    xs = rng.normal(size=(n, nrhs))
+    b = A @ xs
+    bt = torch.tensor(np.asfortranarray(b), device="cuda")
+    bt = bt.t().contiguous().t() if nrhs > 1 else bt[:, 0].contiguous()
+    ldl.solve(bt)
+    x = bt.cpu().numpy().reshape(n, nrhs)
+    r = np.linalg.norm(A @ x - b) / np.linalg.norm(b)
+    return r, x
+
+
+@pytest.mark.parametrize("index", range(7))
+@pytest.mark.parametrize("order", ["NATURAL", "AMD"])
+def test_small_matrices(index, order):
+    A, inertia = M.small(index)
+    _, ldl = _gpu_ldl(A, order, ustab=0.4)
+    assert ldl.get_inertia() == inertia
+    r, _ = _residual(A, ldl)
+    assert r < 1e-13
+
+
+@pytest.mark.parametrize("n", [10, 20, 50])
+@pytest.mark.parametrize("order", ["NATURAL", "AMD"])
+def test_poisson(n, order):
+    ref = _ref()
+    A = M.poisson_2d(n)
+    _, ldl = _gpu_ldl(A, order, ustab=0.4, delay_growth=1.0)
+    R = ref.RefLDL(A.indptr, A.indices, ustab=0.4, delay_growth=1.0, order=order)
+    R.factor(A.data)
+    assert ldl.get_inertia() == R.inertia() == (A.shape[0], 0)
+    r, x = _residual(A, ldl, nrhs=7)
+    assert r < 1e-12
+
+
+@pytest.mark.parametrize(
+    "nx,nc,seed,order",
+    [(300, 120, 3, "AMD"), (300, 120, 3, "NATURAL"), (1500, 700, 4, "AMD"), (2000, 900, 5, "NATURAL")],
+)
+def test_kkt_inertia_and_solve(nx, nc, seed, order):
+    ref = _ref()
+    A = M.kkt(nx, nc, seed)
+    _, ldl = _gpu_ldl(A, order)
+    R = ref.RefLDL(A.indptr, A.indices, order=order)
+    R.factor(A.data)
+    assert ldl.get_inertia() == R.inertia() == (nx, nc)
+    r, _ = _residual(A, ldl)
+    assert r < 1e-10
+    np_g, nd_g = ldl.pivot_counts()
+    np_r, nd_r = R.pivot_counts()
+    # The same threshold tests run on values that differ only by rounding.
+    assert np.sum(nd_g != nd_r) <= max(2, len(nd_r) // 100)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_indefinite_hessian_kkt(seed):
+    ref = _ref()
+    A = M.kkt(400, 150, seed, indefinite_h=True)
+    _, ldl = _gpu_ldl(A, "AMD")
+    R = ref.RefLDL(A.indptr, A.indices, order="AMD")
+    R.factor(A.data)
+    ev = np.linalg.eigvalsh(A.toarray())
+    assert ldl.get_inertia() == R.inertia() == (int(np.sum(ev >= 0)), int(np.sum(ev < 0)))
+    r, _ = _residual(A, ldl)
+    assert r < 1e-9
+
+
+@pytest.mark.parametrize("side", [16, 40])
+def test_grid_kkt(side):
+    ref = _ref()
+    A, npr, ncon = M.grid_kkt(side, seed=side)
+    _, ldl = _gpu_ldl(A, "AMD")
+    R = ref.RefLDL(A.indptr, A.indices, order="AMD")
+    R.factor(A.data)
+    assert ldl.get_inertia() == R.inertia()
+    r, _ = _residual(A, ldl)
+    assert r < 1e-10
+
+
+def test_refactor_same_handle():
+    A = M.kkt(500, 200, 7)
+    mat, ldl = _gpu_ldl(A, "AMD")
+    for s in (1.0, 2.0, 0.5):
+        mat.data.copy_(torch.tensor(A.data * s, device="cuda"))
+        ldl.factor()
+        assert ldl.get_inertia() == (500, 200)
+        As = A * s
+        r, _ = _residual(As, ldl)
+        assert r < 1e-10
+
+
+def test_shifted_factor():
+    A = M.kkt(300, 100, 9, sigma=0.0, indefinite_h=True)
+    mat, ldl = _gpu_ldl(A, "AMD")
+    n = A.shape[0]
+    shift = np.zeros(n)
+    shift[:300] = 10.0
+    shift[300:] = -1e-3
+    ldl.factor(shift=torch.tensor(shift, device="cuda"))
+    from scipy.sparse import diags
+
+    B = (A + diags(shift)).tocsr()
+    ev = np.linalg.eigvalsh(B.toarray())
+    assert ldl.get_inertia() == (int(np.sum(ev >= 0)), int(np.sum(ev < 0)))
+    r, _ = _residual(B, ldl)
+    assert r < 1e-10
