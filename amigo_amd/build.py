@@ -20,6 +20,8 @@ LIB = os.path.join(HERE, "libamigo_amd.so")
 SOURCES = [
     "symbolic.cpp",
     "ldl_numeric.hip",
+    "ipm_kernels.hip",
+    "assembly.hip",
     "capi.cpp",
 ]
 

@@ -1,0 +1,50 @@
+/*
+ * amigo_amd — KKT gradient / Hessian assembly of component groups (C ABI).
+ *
+ * Replaces OptimizationProblem::gradient / hessian / lagrangian
+ * (include/optimization_problem.h:718-818) for the element types below;
+ * device pointers throughout, results stream-ordered.
+ */
+#ifndef AMIGO_ASSEMBLY_H
+#define AMIGO_ASSEMBLY_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  AMIGO_ELEM_LINK = 0,        /* (x, lambda): lambda * x            */
+  AMIGO_ELEM_DI_INTERVAL = 1, /* double-integrator collocation interval */
+  AMIGO_ELEM_ELLIPTIC2D = 2,  /* 5-point semilinear elliptic control node */
+  AMIGO_ELEM_ELLIPTIC3D = 3,  /* 7-point semilinear elliptic control node */
+  AMIGO_ELEM_CARTPOLE = 4     /* cart-pole collocation interval */
+};
+
+const char* amigo_assembly_last_error(void);
+
+/* Number of inputs per element of a type (-1 if unknown). */
+int amigo_elem_num_inputs(int type);
+
+/* Per-element gradient (E x NC) and Hessian (E x NC x NC, row-major) of
+ * alpha * f_e + lambda^T c_e at x; idx is E x NC (-1 = absent input), data
+ * is E x ndata.  gbuf / hbuf / lpart may be NULL; lpart receives one partial
+ * Lagrangian sum per block of 256 elements. */
+int amigo_elements_eval(int type, int E, const int* idx, const double* data,
+                        int ndata, const double* x, double alpha, double* gbuf,
+                        double* hbuf, double* lpart, void* stream);
+
+/* out[i] (+)= sum over p in [ptr[i], ptr[i+1]) of buf[src[p]], summed in
+ * list order (deterministic). */
+int amigo_gather_sum(int n, const int* ptr, const long long* src,
+                     const double* buf, double* out, int accumulate,
+                     void* stream);
+
+/* out_dev[0] = sum of part[0..nb) in order; copied to out_host if given. */
+int amigo_sum_partials(const double* part, int nb, double* out_dev,
+                       double* out_host, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AMIGO_ASSEMBLY_H */
