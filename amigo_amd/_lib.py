@@ -65,6 +65,21 @@ _SIGS = {
             ctypes.POINTER(ctypes.c_void_p),
         ],
     ),
+    "amigo_ldl_create_grouped": (
+        ctypes.c_int,
+        [
+            ctypes.c_int,
+            _c_int_p,
+            _c_int_p,
+            ctypes.c_int,
+            ctypes.c_double,
+            ctypes.c_double,
+            ctypes.c_double,
+            ctypes.c_int,
+            _c_int_p,
+            ctypes.POINTER(ctypes.c_void_p),
+        ],
+    ),
     "amigo_ldl_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "amigo_ldl_sizes": (ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
     "amigo_ldl_get_symbolic": (
@@ -83,7 +98,78 @@ _SIGS = {
         [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
     ),
     "amigo_ldl_work": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p, _c_dbl_p]),
+    "amigo_ldl_get_stats": (ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p]),
+    "amigo_ldl_set_scaling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "amigo_ldl_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "amigo_ldl_get_timing": (
+        ctypes.c_int,
+        [ctypes.c_void_p, _c_dbl_p, ctypes.POINTER(ctypes.c_longlong), _c_dbl_p, _c_dbl_p],
+    ),
 }
+
+
+class IpmProblem(ctypes.Structure):
+    """amigo_ipm_problem (include/amigo_ipm.h)."""
+
+    _fields_ = [
+        ("n_primal", ctypes.c_int),
+        ("n_constraints", ctypes.c_int),
+        ("primal_indices", ctypes.c_void_p),
+        ("constraint_indices", ctypes.c_void_p),
+        ("lbx", ctypes.c_void_p),
+        ("ubx", ctypes.c_void_p),
+        ("lbh", ctypes.c_void_p),
+    ]
+
+
+class IpmState(ctypes.Structure):
+    """amigo_ipm_state (include/amigo_ipm.h)."""
+
+    _fields_ = [
+        ("xlam", ctypes.c_void_p),
+        ("zl", ctypes.c_void_p),
+        ("zu", ctypes.c_void_p),
+        ("sl", ctypes.c_void_p),
+        ("su", ctypes.c_void_p),
+    ]
+
+
+_V = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_PP = ctypes.POINTER(IpmProblem)
+_PS = ctypes.POINTER(IpmState)
+_SIGS.update(
+    {
+        "amigo_ipm_last_error": (ctypes.c_char_p, []),
+        "amigo_ipm_workspace_doubles": (_I, []),
+        "amigo_ipm_residual": (_I, [_PP, _PS, _D, _V, _V, _V, _c_dbl_p, _V]),
+        "amigo_ipm_diagonal": (_I, [_PP, _PS, _V, _V]),
+        "amigo_ipm_bound_dual_step": (_I, [_PP, _PS, _D, _V, _V, _V, _V]),
+        "amigo_ipm_max_step": (_I, [_PP, _PS, _D, _V, _V, _V, _V, _c_dbl_p, _V]),
+        "amigo_ipm_apply_step": (_I, [_PP, _PS, _D, _D, _V, _V, _V, _I, _PS, _V]),
+        "amigo_ipm_reduce": (_I, [_I, _PP, _PS, _D, _V, _V, _V, _V, _c_dbl_p, _V]),
+        "amigo_ipm_reset_bound_multipliers": (_I, [_PP, _PS, _D, _D, _V]),
+        "amigo_ipm_dual_residual_vector": (_I, [_PP, _PS, _V, _V, _I, _V]),
+        "amigo_ipm_initialize": (_I, [_V, _V, _I, _V, _D, _D, _D, _D, _V, _V, _PS, _I, _I, _V]),
+        "amigo_ipm_floor_slacks": (_I, [_PP, _PS, _V, _V, _D, _D, _D, _V, _c_dbl_p, _V]),
+        "amigo_csr_spmv": (_I, [_I, _V, _V, _V, _V, _V, _V]),
+        "amigo_csr_scale_symmetric": (_I, [_I, _V, _V, _V, _V, _V]),
+        "amigo_ipm_constraint_scaling": (_I, [_I, _V, _V, _V, _D, _D, _V, _V]),
+        "amigo_ipm_scale_rows": (_I, [_I, _V, _V, _V, _I, _V]),
+        "amigo_ipm_ir_init": (_I, [_PP, _PS, _D, _V, _V, _V, _V, _V, _V, _c_dbl_p, _V]),
+        "amigo_ipm_ir_residual": (
+            _I,
+            [_PP, _PS, _D, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V, _c_dbl_p, _V],
+        ),
+        "amigo_ipm_ir_update": (_I, [_PP, _PS, _I, _V, _V, _V, _V, _V, _V, _V]),
+        "amigo_assembly_last_error": (ctypes.c_char_p, []),
+        "amigo_elem_num_inputs": (_I, [_I]),
+        "amigo_elements_eval": (_I, [_I, _I, _V, _V, _I, _V, _D, _V, _V, _V, _V]),
+        "amigo_gather_sum": (_I, [_I, _V, _V, _V, _V, _I, _V]),
+        "amigo_sum_partials": (_I, [_V, _I, _V, _c_dbl_p, _V]),
+    }
+)
 
 
 def exported_names():
@@ -116,8 +202,18 @@ def lib():
     return _lib
 
 
-def check(rc):
+def check(rc, which="ldl"):
     if rc != 0:
-        msg = lib().amigo_last_error()
+        fn = {
+            "ldl": "amigo_last_error",
+            "ipm": "amigo_ipm_last_error",
+            "asm": "amigo_assembly_last_error",
+        }[which]
+        msg = getattr(lib(), fn)()
         raise AmigoError(rc, msg.decode() if msg else "")
     return rc
+
+
+def dptr(t):
+    """void* of a device tensor (None -> NULL)."""
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)

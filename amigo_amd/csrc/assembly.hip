@@ -96,31 +96,34 @@ __device__ void eval_di(const double* z, const double* d, double alpha, double& 
 }
 
 // AMIGO_ELEM_ELLIPTIC2D / 3D: one grid node of -Lap(y) + y^3 - u = 0 with the
-// tracking objective.  inputs (y, y_nb[2*dim], u, lambda); data (h, yd, a).
-// f_e = h^dim/2 (y - yd)^2 + a h^dim/2 u^2,  c = (2 dim y - sum y_nb)/h^2 + y^3 - u
+// tracking objective, the constraint multiplied by h^2 (the usual
+// finite-difference scaling: Jacobian entries O(1)).
+// inputs (y, y_nb[2*dim], u, lambda); data (h, yd, a).
+// f_e = h^dim/2 (y - yd)^2 + a h^dim/2 u^2
+// c   = (2 dim y - sum y_nb) + h^2 (y^3 - u)
 template <int DIM>
 __device__ void eval_elliptic(const double* z, const double* d, double alpha, double& L,
                               double* g, double* H) {
   constexpr int NB = 2 * DIM, NC = NB + 3;
   const double h = d[0], yd = d[1], a = d[2];
   const double hd_ = DIM == 2 ? h * h : h * h * h;
-  const double ih2 = 1.0 / (h * h);
+  const double h2 = h * h;
   const double y = z[0], u = z[NB + 1], lam = z[NB + 2];
   double s = 0.0;
   for (int q = 0; q < NB; q++) s += z[1 + q];
-  const double c = (2.0 * DIM * y - s) * ih2 + y * y * y - u;
+  const double c = (2.0 * DIM * y - s) + h2 * (y * y * y - u);
   L = alpha * (0.5 * hd_ * (y - yd) * (y - yd) + 0.5 * a * hd_ * u * u) + lam * c;
-  const double dcy = 2.0 * DIM * ih2 + 3.0 * y * y;
+  const double dcy = 2.0 * DIM + 3.0 * h2 * y * y;
   g[0] = alpha * hd_ * (y - yd) + lam * dcy;
-  for (int q = 0; q < NB; q++) g[1 + q] = -lam * ih2;
-  g[NB + 1] = alpha * a * hd_ * u - lam;
+  for (int q = 0; q < NB; q++) g[1 + q] = -lam;
+  g[NB + 1] = alpha * a * hd_ * u - lam * h2;
   g[NB + 2] = c;
   for (int q = 0; q < NC * NC; q++) H[q] = 0.0;
-  H[0] = alpha * hd_ + 6.0 * lam * y;
+  H[0] = alpha * hd_ + 6.0 * lam * h2 * y;
   H[(NB + 1) * NC + NB + 1] = alpha * a * hd_;
   H[(NB + 2) * NC + 0] = H[0 * NC + NB + 2] = dcy;
-  for (int q = 0; q < NB; q++) H[(NB + 2) * NC + 1 + q] = H[(1 + q) * NC + NB + 2] = -ih2;
-  H[(NB + 2) * NC + NB + 1] = H[(NB + 1) * NC + NB + 2] = -1.0;
+  for (int q = 0; q < NB; q++) H[(NB + 2) * NC + 1 + q] = H[(1 + q) * NC + NB + 2] = -1.0;
+  H[(NB + 2) * NC + NB + 1] = H[(NB + 1) * NC + NB + 2] = -h2;
 }
 
 // Cart-pole accelerations (point-mass pole of length l on a cart):

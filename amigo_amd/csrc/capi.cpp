@@ -45,6 +45,7 @@ struct amigo_ldl {
   SymbolicPlan plan;
   LdlNumeric num;
   bool device_ready = false;
+  int scaling = 0;
   std::vector<int> rowp, cols;
 };
 
@@ -110,6 +111,14 @@ int amigo_etree_post_colcount(int n, const int* rowp, const int* cols,
 int amigo_ldl_create(int n, const int* rowp, const int* cols, int solver_type,
                      double ustab, double pivot_tol, double delay_growth,
                      int order, amigo_ldl** out) {
+  return amigo_ldl_create_grouped(n, rowp, cols, solver_type, ustab, pivot_tol,
+                                  delay_growth, order, nullptr, out);
+}
+
+int amigo_ldl_create_grouped(int n, const int* rowp, const int* cols,
+                             int solver_type, double ustab, double pivot_tol,
+                             double delay_growth, int order,
+                             const int* var_group, amigo_ldl** out) {
   if (!out) return fail(AMIGO_ERR_ARG, "amigo_ldl_create: null output");
   *out = nullptr;
   if (!check_csr(n, rowp, cols))
@@ -138,7 +147,12 @@ int amigo_ldl_create(int n, const int* rowp, const int* cols, int solver_type,
   h->delay_growth = delay_growth;
   h->order = order;
   try {
-    symbolic_analysis(order, n, rowp, cols, h->plan);
+    if (var_group)
+      for (int i = 0; i < n; i++)
+        if (var_group[i] < 0 || var_group[i] >= n)
+          return fail(AMIGO_ERR_ARG, "variable group id out of range");
+    symbolic_analysis(order, n, rowp, cols, h->plan,
+                      order == AMIGO_ORDER_NESTED_DISSECTION ? var_group : nullptr);
   } catch (const std::exception& e) {
     return fail(AMIGO_ERR_NOMEM, std::string("symbolic analysis: ") + e.what());
   }
@@ -256,11 +270,56 @@ int amigo_ldl_solve(amigo_ldl* h, int nrhs, double* x_dev, int ldx,
   return AMIGO_OK;
 }
 
+int amigo_ldl_set_scaling(amigo_ldl* h, int mode) {
+  if (!h || mode < 0 || mode > 1) return fail(AMIGO_ERR_ARG, "amigo_ldl_set_scaling: bad argument");
+  h->scaling = mode;
+  h->num.set_scaling(mode);
+  return AMIGO_OK;
+}
+
+int amigo_ldl_set_timing(amigo_ldl* h, int on) {
+  if (!h) return fail(AMIGO_ERR_ARG, "null handle");
+  h->num.set_timing(on != 0);
+  if (on == 2) h->num.reset_timing();
+  return AMIGO_OK;
+}
+
+int amigo_ldl_get_timing(const amigo_ldl* h, double* ms5, long long* count5,
+                         double* flops_small, double* flops_big) {
+  if (!h || !ms5 || !count5) return fail(AMIGO_ERR_ARG, "null argument");
+  h->num.get_timing(ms5, count5);
+  if (flops_small) *flops_small = h->num.flops_small();
+  if (flops_big) *flops_big = h->num.flops_big();
+  return AMIGO_OK;
+}
+
+int amigo_ldl_get_stats(const amigo_ldl* h, int* num_delayed, int* max_front) {
+  if (!h) return fail(AMIGO_ERR_ARG, "null handle");
+  if (!h->num.factored()) return fail(AMIGO_ERR_STATE, "no factorization");
+  int d = 0, m = 0;
+  h->num.stats(&d, &m);
+  if (num_delayed) *num_delayed = d;
+  if (max_front) *max_front = m;
+  return AMIGO_OK;
+}
+
 int amigo_ldl_work(const amigo_ldl* h, double* factor_flops,
                    double* solve_bytes) {
   if (!h) return fail(AMIGO_ERR_ARG, "null handle");
-  if (factor_flops) *factor_flops = h->num.factor_flops();
-  if (solve_bytes) *solve_bytes = h->num.solve_bytes();
+  // Algorithmic counts from the symbolic, no-delay structure.
+  const SymbolicPlan& P = h->plan;
+  double f = 0.0, b = 0.0;
+  for (int k = 0; k < P.num_nodes; k++) {
+    const double ns = P.node_ptr[k + 1] - P.node_ptr[k];
+    const double m = ns + (P.contrib_ptr[k + 1] - P.contrib_ptr[k]);
+    for (int q = 0; q < (int)ns; q++) {
+      const double r = m - q - 1;
+      f += r + r * (m - q);
+    }
+    b += 8.0 * 2.0 * m * ns + 8.0 * 2.0 * m + 4.0 * m;
+  }
+  if (factor_flops) *factor_flops = f;
+  if (solve_bytes) *solve_bytes = b;
   return AMIGO_OK;
 }
 

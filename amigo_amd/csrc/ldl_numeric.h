@@ -38,7 +38,7 @@ struct DevState {
   int* m = nullptr;        // [K] front size
   int64_t* foff = nullptr; // [K] offset of the front in rpool (doubles)
   int64_t* voff = nullptr; // [K] offset of the front variables in ipool
-  int* pstate = nullptr;   // [K*4] blocked-path state: k, nc, done, unused
+  int* pstate = nullptr;   // [K*8] blocked path: k, nc, kstart, kw, blocked
   double* rpool = nullptr;
   int* ipool = nullptr;
   unsigned long long* tops = nullptr;  // [2] bump pointers (rpool, ipool)
@@ -50,6 +50,9 @@ struct FactorParams {
   double ustab;
   double pivot_tol;
 };
+
+// Kernel classes timed with HIP events when timing is on.
+enum TimedKernel { TK_LEVEL = 0, TK_PANEL = 1, TK_TRAILING = 2, TK_SOLVE_FWD = 3, TK_SOLVE_BWD = 4, TK_COUNT = 5 };
 
 class LdlNumeric {
  public:
@@ -69,15 +72,41 @@ class LdlNumeric {
     *zero = inertia_[2];
   }
   int pivot_counts(int* np, int* nd, std::string& err) const;
+  void stats(int* delayed, int* max_front) const {
+    *delayed = delayed_;
+    *max_front = max_front_seen_;
+  }
   double factor_flops() const { return flops_; }
   double solve_bytes() const { return solve_bytes_; }
   int num_levels() const { return int(level_ptr_.size()) - 1; }
   int max_front() const { return max_front_; }
   bool factored() const { return factored_; }
+  // 0: none (the reference's behaviour); 1: symmetric equilibration of
+  // the shifted matrix before the factorization (inertia is unchanged).
+  void set_scaling(int mode) { scaling_ = mode != 0; }
+  // HIP-event timing of every launch, accumulated per kernel class.
+  void set_timing(bool on) { timing_ = on; }
+  void get_timing(double* ms, long long* counts) const {
+    for (int c = 0; c < TK_COUNT; c++) {
+      ms[c] = tk_ms_[c];
+      counts[c] = tk_n_[c];
+    }
+  }
+  void reset_timing() {
+    for (int c = 0; c < TK_COUNT; c++) {
+      tk_ms_[c] = 0.0;
+      tk_n_[c] = 0;
+    }
+  }
+  // Algorithmic flops of the fronts factored by the workgroup kernel
+  // (static size <= the LDS limit) and by the blocked path.
+  double flops_small() const { return flops_small_; }
+  double flops_big() const { return flops_big_; }
 
  private:
   int run_factor(const double* vals, const double* shift, FactorParams prm,
-                 bool cholesky, hipStream_t stream, std::string& err);
+                 bool cholesky, bool sync_mode, hipStream_t stream,
+                 std::string& err);
   int ensure_pools(unsigned long long rcap, unsigned long long icap,
                    std::string& err);
 
@@ -90,6 +119,16 @@ class LdlNumeric {
   std::vector<int> level_is_big_;   // per level: has big fronts
   std::vector<int> big_nodes_;      // per level list of big fronts
   std::vector<int> big_ptr_;
+  std::vector<int> big_npanels_, big_tiles_;  // static launch plan per level
+  const int* d_big_nodes_ = nullptr;
+  const int* d_big_root_ = nullptr;
+  const int* d_big_ns_ = nullptr;
+  const int64_t* d_big_woff_ = nullptr;
+  int64_t* d_big_woff_sync_ = nullptr;
+  int* d_sync_nodes_ = nullptr;
+  int* d_sync_root_ = nullptr;
+  int* d_sync_ns_ = nullptr;
+  int delayed_ = 0, max_front_seen_ = 0;
   std::vector<int> level_lds_;      // LDS doubles per level kernel
   int* d_level_nodes_ = nullptr;
   int* d_big_mask_ = nullptr;       // [K] 1 if front is factored blocked
@@ -103,6 +142,20 @@ class LdlNumeric {
   double flops_ = 0.0, solve_bytes_ = 0.0;
   bool factored_ = false;
   int max_p_ = 0;                   // largest fully summed block seen
+  bool scaling_ = false;
+  bool timing_ = false;
+  double tk_ms_[TK_COUNT] = {0, 0, 0, 0, 0};
+  long long tk_n_[TK_COUNT] = {0, 0, 0, 0, 0};
+  std::vector<hipEvent_t> ev_;
+  std::vector<int> ev_class_;
+  int ev_used_ = 0;
+  double flops_small_ = 0.0, flops_big_ = 0.0;
+  void tk_begin(hipStream_t s);
+  void tk_end(hipStream_t s, int cls);
+  void tk_collect();
+  double* scale_ = nullptr;
+  const int* d_rowp_ = nullptr;
+  const int* d_cols_ = nullptr;
 };
 
 }  // namespace amigo_amd

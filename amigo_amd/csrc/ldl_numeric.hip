@@ -31,6 +31,8 @@ constexpr int NB = 64;           // panel width of the blocked path
 constexpr int LDS_MAX_M = 128;   // largest front held in LDS (128 KiB)
 constexpr int BIG_M = 128;       // static fronts above this use the blocked path
 constexpr int TILE = 64;         // Schur-update tile
+constexpr int DCAP_BIG = 64;     // delayed pivots a blocked front is sized for
+constexpr int PS = 8;            // ints of blocked-path state per front
 
 struct VI {
   double v;
@@ -106,6 +108,7 @@ constexpr double BK_ALPHA = 0.6403882032022076;  // (1 + sqrt(17)) / 8
 // as positive like the reference; `zero` is reported separately.
 __device__ void count_inertia(int step, double d11, double d21, double d22,
                               bool zero, int* cnt) {
+  if (isnan(d11) || isnan(d21) || isnan(d22)) cnt[3]++;
   if (step == 1) {
     if (d11 >= 0.0)
       cnt[0]++;
@@ -150,7 +153,8 @@ __device__ int factor_unblocked(double* F, int ld, int m, int nfs, int* vars,
       VI ca = block_argmax(a, red);
       VI cb = block_argmax(b, red);
       const double akk = fabs(F[k + (size_t)k * ld]);
-      const bool has_cand = k + 1 < nc;
+      // A column of NaNs selects no row: treat it as having no candidate.
+      const bool has_cand = k + 1 < nc && ca.i > k && ca.i < nc;
       const int imax = has_cand ? ca.i : 0;
       const double abswik = has_cand ? ca.v : 0.0;
       if (!bk) {
@@ -280,6 +284,187 @@ __device__ int factor_unblocked(double* F, int ld, int m, int nfs, int* vars,
   return k;
 }
 
+// Column statistics of column k in one pass: the largest candidate entry
+// (rows k+1..nc-1, with its row) and the largest contribution entry (rows
+// nc..m-1).  The column is cached in Wv on the way.  Two barriers.
+struct ColStats {
+  double cv;
+  int ci;
+  double bv;
+};
+
+__device__ ColStats block_colstats(const double* F, int ld, int m, int k, int nc,
+                                   double* Wv, ColStats* red) {
+  ColStats x = {-1.0, INT_MAX, -1.0};
+  for (int i = k + 1 + threadIdx.x; i < m; i += NT) {
+    const double w = F[i + (size_t)k * ld];
+    Wv[i] = w;
+    const double a = fabs(w);
+    if (i < nc) {
+      if (a > x.cv || (a == x.cv && i < x.ci)) {
+        x.cv = a;
+        x.ci = i;
+      }
+    } else {
+      x.bv = fmax(x.bv, a);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(x.cv, off, 64);
+    const int oi = __shfl_xor(x.ci, off, 64);
+    const double ob = __shfl_xor(x.bv, off, 64);
+    if (ov > x.cv || (ov == x.cv && oi < x.ci)) {
+      x.cv = ov;
+      x.ci = oi;
+    }
+    x.bv = fmax(x.bv, ob);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = x;
+  __syncthreads();
+  ColStats r = red[0];
+#pragma unroll
+  for (int w = 1; w < NT / 64; w++) {
+    const ColStats o = red[w];
+    if (o.cv > r.cv || (o.cv == r.cv && o.ci < r.ci)) {
+      r.cv = o.cv;
+      r.ci = o.ci;
+    }
+    r.bv = fmax(r.bv, o.bv);
+  }
+  return r;
+}
+
+// LDS-resident front: same pivot rules as factor_unblocked, organized for
+// few barriers.  The common 1x1 step without interchange costs three
+// barriers: column statistics (the column is cached in Wv) and one fused
+// rank-1 update + scaling pass.
+__device__ int factor_front_lds(double* F, int ld, int m, int nfs, int* vars,
+                                bool bk, bool chol, double ustab, double pivot_tol,
+                                double* Wv, VI* red, ColStats* cred, int* cnt) {
+  const int tid = threadIdx.x;
+  int k = 0, nc = nfs;
+  while (k < nc) {
+    int step = 1, kpiv = k;
+    bool zero = false;
+    const ColStats st = block_colstats(F, ld, m, k, nc, Wv, cred);
+    if (!chol) {
+      const double akk = fabs(F[k + (size_t)k * ld]);
+      const bool has_cand = k + 1 < nc && st.ci > k && st.ci < nc;
+      const int imax = has_cand ? st.ci : 0;
+      const double abswik = has_cand ? st.cv : 0.0;
+      const double colmax = bk ? abswik : fmax(abswik, nc < m ? st.bv : 0.0);
+      const bool ok1 = bk ? (fmax(akk, colmax) == 0.0 || akk >= BK_ALPHA * colmax)
+                          : (akk >= pivot_tol && akk >= ustab * colmax);
+      if (ok1) {
+        zero = bk && fmax(akk, colmax) == 0.0;
+      } else if (has_cand) {
+        VI r = {-1.0, INT_MAX};
+        for (int i = k + tid; i < m; i += NT) {
+          if (i == imax) continue;
+          const double g = i < imax ? F[imax + (size_t)i * ld] : F[i + (size_t)imax * ld];
+          r = vi_best(r, VI{fabs(g), i});
+        }
+        const double rowmax = block_argmax(r, red).v;
+        const double aii = fabs(F[imax + (size_t)imax * ld]);
+        if (bk) {
+          if (akk >= BK_ALPHA * colmax * (colmax / rowmax)) {
+            step = 1;
+            kpiv = k;
+          } else if (aii >= BK_ALPHA * rowmax) {
+            step = 1;
+            kpiv = imax;
+          } else {
+            step = 2;
+            kpiv = imax;
+          }
+        } else if (aii >= pivot_tol && aii >= ustab * rowmax) {
+          step = 1;
+          kpiv = imax;
+        } else {
+          const double wkk = F[k + (size_t)k * ld];
+          const double wik = F[imax + (size_t)k * ld];
+          const double wii = F[imax + (size_t)imax * ld];
+          const double det = fabs(wkk * wii - wik * wik);
+          step = 0;
+          if (det >= pivot_tol) {
+            const double delta = det / fmax(aii + abswik, akk + abswik);
+            if (delta >= ustab * colmax && delta >= ustab * rowmax) {
+              step = 2;
+              kpiv = imax;
+            }
+          }
+        }
+      } else {
+        step = 0;
+      }
+    }
+    if (step == 0) {
+      if (k < nc - 1) symm_swap(F, ld, m, vars, k, nc - 1);
+      nc -= 1;
+      continue;
+    }
+    const int ksrc = k + step - 1;
+    const bool swapped = kpiv != ksrc;
+    if (swapped) symm_swap(F, ld, m, vars, ksrc, kpiv);
+    if (step == 1) {
+      const double d = F[k + (size_t)k * ld];
+      if (swapped) {
+        for (int i = k + 1 + tid; i < m; i += NT) Wv[i] = F[i + (size_t)k * ld];
+        __syncthreads();
+      }
+      if (!zero) {
+        const double dinv = 1.0 / d;
+        const int w = m - k - 1;
+        const int tot = w * w;
+        for (int e = tid; e < tot; e += NT) {
+          const int ii = e % w, jj = e / w;
+          if (ii < jj) continue;
+          const int i = k + 1 + ii, j = k + 1 + jj;
+          F[i + (size_t)j * ld] -= (Wv[i] * dinv) * Wv[j];
+        }
+        for (int i = k + 1 + tid; i < m; i += NT) F[i + (size_t)k * ld] = Wv[i] * dinv;
+      }
+      if (tid == 0) count_inertia(1, d, 0.0, 0.0, zero, cnt);
+      __syncthreads();
+      k += 1;
+    } else {
+      const double a = F[k + (size_t)k * ld];
+      const double b = F[k + 1 + (size_t)k * ld];
+      const double c = F[k + 1 + (size_t)(k + 1) * ld];
+      const double detinv = 1.0 / (a * c - b * b);
+      const double i00 = c * detinv, i01 = -b * detinv, i11 = a * detinv;
+      const int w = m - k - 2;
+      const int tot = w * w;
+      for (int e = tid; e < tot; e += NT) {
+        const int ii = e % w, jj = e / w;
+        if (ii < jj) continue;
+        const int i = k + 2 + ii, j = k + 2 + jj;
+        const double w1 = F[i + (size_t)k * ld], w2 = F[i + (size_t)(k + 1) * ld];
+        const double l1 = w1 * i00 + w2 * i01;
+        const double l2 = w1 * i01 + w2 * i11;
+        F[i + (size_t)j * ld] -= l1 * F[j + (size_t)k * ld] + l2 * F[j + (size_t)(k + 1) * ld];
+      }
+      __syncthreads();
+      for (int i = k + 2 + tid; i < m; i += NT) {
+        const double w1 = F[i + (size_t)k * ld], w2 = F[i + (size_t)(k + 1) * ld];
+        F[i + (size_t)k * ld] = w1 * i00 + w2 * i01;
+        F[i + (size_t)(k + 1) * ld] = w1 * i01 + w2 * i11;
+      }
+      if (tid == 0) {
+        vars[k] = -(vars[k] + 1);
+        vars[k + 1] = -(vars[k + 1] + 1);
+        count_inertia(2, a, b, c, false, cnt);
+      }
+      __syncthreads();
+      k += 2;
+    }
+  }
+  return k;
+}
+
 // Position of row r of child `c`'s contribution block inside the parent's
 // front.  Rows [0, dc) are the child's delayed pivots, placed at
 // doff + r; the rest are static rows mapped through cmap.
@@ -296,11 +481,12 @@ __global__ __launch_bounds__(NT) void level_kernel(
     DevPlan P, DevState S, const double* __restrict__ vals,
     const double* __restrict__ shift, const int* __restrict__ lnodes,
     const int* __restrict__ big_mask, int lds_doubles, double ustab,
-    double pivot_tol, int chol) {
-  extern __shared__ double smem[];
+    double pivot_tol, int chol, int dcap_big, const double* __restrict__ sc) {
+  extern __shared__ double smem[];  // F (lds_doubles) then the cached column
   __shared__ VI red[NT / 64];
+  __shared__ ColStats cred[NT / 64];
   __shared__ int sh[4];
-  __shared__ int cnt[3];
+  __shared__ int cnt[4];
   const int tid = threadIdx.x;
   const int node = lnodes[blockIdx.x];
   const int ns = P.ns[node], cs = P.cs[node];
@@ -308,16 +494,24 @@ __global__ __launch_bounds__(NT) void level_kernel(
 
   if (tid == 0) {
     sh[0] = -1;
-    cnt[0] = cnt[1] = cnt[2] = 0;
+    cnt[0] = cnt[1] = cnt[2] = cnt[3] = 0;
   }
   if (tid == 0 && atomicAdd(&S.flags[0], 0) == 0) {
     int D = 0;
     for (int q = kb; q < ke; q++) D += S.ndel[P.kids[q]];
     const int m = ns + D + cs;
+    if (dcap_big >= 0 && m > LDS_MAX_M && (!big_mask[node] || D > dcap_big)) {
+      // The statically planned blocked launches cannot hold this front:
+      // the factorization is redone with per-level front sizes.
+      atomicExch(&S.flags[0], -7);
+      D = -1;
+    }
     const unsigned long long need = (unsigned long long)m * (unsigned long long)m;
     const unsigned long long fo = atomicAdd(&S.tops[0], need);
     const unsigned long long vo = atomicAdd(&S.tops[1], (unsigned long long)m);
-    if (fo + need > S.rcap || vo + m > S.icap) {
+    if (D < 0) {
+      sh[0] = -1;
+    } else if (fo + need > S.rcap || vo + m > S.icap) {
       atomicExch(&S.flags[0], -4);
       sh[0] = -1;
     } else {
@@ -334,7 +528,9 @@ __global__ __launch_bounds__(NT) void level_kernel(
   const int D = sh[0], m = sh[1];
   double* G = S.rpool + S.foff[node];
   int* vars = S.ipool + S.voff[node];
-  const bool big = big_mask[node] != 0;
+  // Fronts larger than the LDS limit go to the blocked path (decided on the
+  // actual size, which includes delayed pivots).
+  const bool big = m > LDS_MAX_M;
   const bool in_lds = !big && (long long)m * m <= lds_doubles;
   double* F = in_lds ? smem : G;
   const int ld = m;
@@ -362,7 +558,8 @@ __global__ __launch_bounds__(NT) void level_kernel(
     int r = P.arow[t];
     r = r < ns ? r : r + D;
     const int c = P.acol[t];
-    const double v = vals[P.aidx[t]];
+    double v = vals[P.aidx[t]];
+    if (sc) v *= sc[vars[r]] * sc[vars[c]];
     if (r >= c)
       F[r + (size_t)c * ld] += v;
     else
@@ -370,7 +567,10 @@ __global__ __launch_bounds__(NT) void level_kernel(
   }
   __syncthreads();
   if (shift != nullptr) {
-    for (int j = tid; j < ns; j += NT) F[j + (size_t)j * ld] += shift[vars[j]];
+    for (int j = tid; j < ns; j += NT) {
+      const int v = vars[j];
+      F[j + (size_t)j * ld] += sc ? shift[v] * sc[v] * sc[v] : shift[v];
+    }
     __syncthreads();
   }
 
@@ -401,25 +601,41 @@ __global__ __launch_bounds__(NT) void level_kernel(
 
   if (big) {
     if (tid == 0) {
-      S.pstate[4 * node + 0] = 0;
-      S.pstate[4 * node + 1] = ns + D;
-      S.pstate[4 * node + 2] = 0;
-      S.pstate[4 * node + 3] = 0;
+      S.pstate[PS * node + 0] = 0;
+      S.pstate[PS * node + 1] = ns + D;
+      S.pstate[PS * node + 2] = 0;
+      S.pstate[PS * node + 3] = 0;
+      S.pstate[PS * node + 4] = 1;
     }
     return;
   }
 
-  const int np = factor_unblocked(F, ld, m, ns + D, vars, cs == 0, chol != 0,
-                                  ustab, pivot_tol, red, cnt);
+  int np;
+  if (in_lds) {
+    np = factor_front_lds(F, ld, m, ns + D, vars, cs == 0, chol != 0, ustab,
+                          pivot_tol, smem + lds_doubles, red, cred, cnt);
+  } else {
+    np = factor_unblocked(F, ld, m, ns + D, vars, cs == 0, chol != 0, ustab,
+                          pivot_tol, red, cnt);
+  }
   if (in_lds) {
     for (size_t e = tid; e < mm; e += NT) G[e] = F[e];
   }
   if (tid == 0) {
     S.npiv[node] = np;
     S.ndel[node] = ns + D - np;
+    // A planned blocked front that fit in a workgroup: the panel launches
+    // of its level find nothing to do.
+    S.pstate[PS * node + 0] = np;
+    S.pstate[PS * node + 1] = np;
+    S.pstate[PS * node + 3] = 0;
+    S.pstate[PS * node + 4] = 0;
+    if (ns + D - np) atomicAdd(&S.flags[5], ns + D - np);
+    atomicMax(&S.flags[6], m);
     if (cnt[0]) atomicAdd(&S.flags[1], cnt[0]);
     if (cnt[1]) atomicAdd(&S.flags[2], cnt[1]);
     if (cnt[2]) atomicAdd(&S.flags[3], cnt[2]);
+    if (cnt[3]) atomicAdd(&S.flags[4], cnt[3]);
   }
 }
 
@@ -427,25 +643,30 @@ __global__ __launch_bounds__(NT) void level_kernel(
 // Blocked path for large fronts.  One panel: up to NB-1 pivots selected with
 // the reference's left-looking panel loop (sparse_ldl.h:1309-1505), W holds
 // the unscaled updated columns (m x NB, ld = m).
-__global__ __launch_bounds__(NT) void panel_kernel(DevState S, int node,
-                                                   double* __restrict__ W,
+__global__ __launch_bounds__(NT) void panel_kernel(DevState S,
+                                                   const int* __restrict__ bnodes,
+                                                   const int* __restrict__ broot,
+                                                   double* __restrict__ Wbase,
+                                                   const int64_t* __restrict__ woff,
                                                    double ustab,
-                                                   double pivot_tol, int root,
-                                                   int chol) {
+                                                   double pivot_tol, int chol) {
+  const int node = bnodes[blockIdx.x];
+  const int root = broot[blockIdx.x];
+  double* W = Wbase + woff[blockIdx.x];
   __shared__ VI red[NT / 64];
-  __shared__ int cnt[3];
+  __shared__ int cnt[4];
   __shared__ int bad;
   const int tid = threadIdx.x;
   if (tid == 0) bad = atomicAdd(&S.flags[0], 0);
   __syncthreads();
   if (bad != 0) return;
-  int* ps = S.pstate + 4 * node;
+  int* ps = S.pstate + PS * node;
   const int m = S.m[node];
   double* F = S.rpool + S.foff[node];
   int* vars = S.ipool + S.voff[node];
   const int ld = m, ldw = m;
   int k = ps[0], nc = ps[1];
-  if (tid == 0) cnt[0] = cnt[1] = cnt[2] = 0;
+  if (tid == 0) cnt[0] = cnt[1] = cnt[2] = cnt[3] = 0;
   __syncthreads();
   if (k >= nc) {
     if (tid == 0) {
@@ -482,7 +703,7 @@ __global__ __launch_bounds__(NT) void panel_kernel(DevState S, int node,
       const VI ca = block_argmax(a, red);
       const VI cb = block_argmax(b, red);
       const double akk = fabs(wk[k]);
-      const bool has_cand = k + 1 < nc;
+      const bool has_cand = k + 1 < nc && ca.i > k && ca.i < nc;
       const int imax = has_cand ? ca.i : 0;
       const double abswik = has_cand ? ca.v : 0.0;
       const double colmax = bk ? abswik : fmax(abswik, nc < m ? cb.v : 0.0);
@@ -596,20 +817,25 @@ __global__ __launch_bounds__(NT) void panel_kernel(DevState S, int node,
     if (cnt[0]) atomicAdd(&S.flags[1], cnt[0]);
     if (cnt[1]) atomicAdd(&S.flags[2], cnt[1]);
     if (cnt[2]) atomicAdd(&S.flags[3], cnt[2]);
+    if (cnt[3]) atomicAdd(&S.flags[4], cnt[3]);
   }
 }
 
 // Trailing update F[k:, k:] -= F[k:, kstart:kstart+kw] W[k:, :kw]^T on the
 // lower triangle, 64x64 tiles (sparse_ldl.h:1172-1197).
-__global__ __launch_bounds__(NT) void trailing_kernel(DevState S, int node,
-                                                      const double* __restrict__ W) {
+__global__ __launch_bounds__(NT) void trailing_kernel(DevState S,
+                                                      const int* __restrict__ bnodes,
+                                                      const double* __restrict__ Wbase,
+                                                      const int64_t* __restrict__ woff) {
+  const int node = bnodes[blockIdx.z];
+  const double* W = Wbase + woff[blockIdx.z];
   __shared__ double As[TILE][NB + 1];
   __shared__ double Bs[TILE][NB + 1];
   __shared__ int bad;
   if (threadIdx.x == 0) bad = atomicAdd(&S.flags[0], 0);
   __syncthreads();
   if (bad != 0) return;
-  const int* ps = S.pstate + 4 * node;
+  const int* ps = S.pstate + PS * node;
   const int m = S.m[node];
   const int k = ps[0], kstart = ps[2], kw = ps[3];
   const int ti = blockIdx.x, tj = blockIdx.y;
@@ -652,11 +878,57 @@ __global__ __launch_bounds__(NT) void trailing_kernel(DevState S, int node,
   }
 }
 
-__global__ void finalize_big_kernel(DevState S, int node, int nfs) {
-  if (S.flags[0] != 0) return;
-  const int nc = S.pstate[4 * node + 1];
+// Close the blocked fronts of a level: pivot/delay counts, and a check that
+// the statically sized panel sequence finished the front (otherwise the
+// factorization is redone level-synchronously).
+__global__ void finalize_big_kernel(DevState S, const int* __restrict__ bnodes,
+                                    const int* __restrict__ bns, int count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count || atomicAdd(&S.flags[0], 0) != 0) return;
+  const int node = bnodes[t];
+  if (S.pstate[PS * node + 4] == 0) return;  // factored in a workgroup
+  const int k = S.pstate[PS * node + 0], nc = S.pstate[PS * node + 1];
+  if (k < nc) {
+    atomicExch(&S.flags[0], -7);
+    return;
+  }
+  const int nfs = bns[t] + S.D[node];
   S.npiv[node] = nc;
   S.ndel[node] = nfs - nc;
+  if (nfs - nc) atomicAdd(&S.flags[5], nfs - nc);
+  atomicMax(&S.flags[6], S.m[node]);
+}
+
+// Symmetric equilibration: sc[i] = 1 / sqrt(max_j |a_ij|) over row i of the
+// shifted matrix (1 for an empty row).  One wave per row.
+__global__ void equilibrate_kernel(int n, const int* __restrict__ rowp,
+                                   const int* __restrict__ cols,
+                                   const double* __restrict__ vals,
+                                   const double* __restrict__ shift,
+                                   double* __restrict__ sc) {
+  const int lane = threadIdx.x & 63;
+  const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nw = (gridDim.x * blockDim.x) >> 6;
+  for (int r = w; r < n; r += nw) {
+    double m = 0.0;
+    for (int p = rowp[r] + lane; p < rowp[r + 1]; p += 64) {
+      double v = vals[p];
+      if (cols[p] == r && shift) v += shift[r];
+      m = fmax(m, fabs(v));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if (lane == 0) sc[r] = m > 0.0 && m < INFINITY ? 1.0 / sqrt(m) : 1.0;
+  }
+}
+
+__global__ void scale_vec_kernel(int n, int nrhs, int ldx, const double* __restrict__ sc,
+                                 double* __restrict__ x) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < (size_t)n * nrhs;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int i = int(t % n), r = int(t / n);
+    x[i + (size_t)r * ldx] *= sc[i];
+  }
 }
 
 __global__ void reset_kernel(DevState S) {
@@ -814,10 +1086,43 @@ int dalloc(std::vector<void*>& allocs, size_t count, T** d, std::string& err) {
 }  // namespace
 
 LdlNumeric::~LdlNumeric() {
+  for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
   for (void* p : allocs_) (void)hipFree(p);
   if (S_.rpool) (void)hipFree(S_.rpool);
   if (S_.ipool) (void)hipFree(S_.ipool);
   if (W_) (void)hipFree(W_);
+}
+
+void LdlNumeric::tk_begin(hipStream_t s) {
+  if (!timing_) return;
+  if (ev_used_ + 2 > (int)ev_.size()) {
+    for (int i = 0; i < 512; i++) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      ev_.push_back(e);
+      ev_class_.push_back(-1);
+    }
+  }
+  (void)hipEventRecord(ev_[ev_used_], s);
+}
+
+void LdlNumeric::tk_end(hipStream_t s, int cls) {
+  if (!timing_ || ev_used_ + 2 > (int)ev_.size()) return;
+  (void)hipEventRecord(ev_[ev_used_ + 1], s);
+  ev_class_[ev_used_] = cls;
+  ev_used_ += 2;
+}
+
+// Called after the stream has been synchronized.
+void LdlNumeric::tk_collect() {
+  for (int i = 0; i + 1 < ev_used_; i += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev_[i], ev_[i + 1]) == hipSuccess) {
+      tk_ms_[ev_class_[i]] += ms;
+      tk_n_[ev_class_[i]] += 1;
+    }
+  }
+  ev_used_ = 0;
 }
 
 int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
@@ -912,34 +1217,54 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   max_front_ = 0;
   double rneed = 0.0, ineed = 0.0;
   flops_ = 0.0;
+  flops_small_ = flops_big_ = 0.0;
   solve_bytes_ = 0.0;
   for (int k = 0; k < K; k++) {
     int m0 = ns[k] + cs[k];
     max_front_ = std::max(max_front_, m0);
-    big[k] = m0 > BIG_M ? 1 : 0;
+    big[k] = m0 + DCAP_BIG > LDS_MAX_M ? 1 : 0;
     rneed += double(m0) * m0;
     ineed += m0;
+    double fk = 0.0;
     for (int q = 0; q < ns[k]; q++) {
       double r = m0 - q - 1;
-      flops_ += r + r * (m0 - q);
+      fk += r + r * (m0 - q);
     }
+    flops_ += fk;
+    if (m0 > LDS_MAX_M)
+      flops_big_ += fk;
+    else
+      flops_small_ += fk;
     solve_bytes_ += 8.0 * 2.0 * double(m0) * ns[k] + 8.0 * 2.0 * m0 + 4.0 * m0;
   }
   level_lds_.assign(nlev, 0);
   big_ptr_.assign(nlev + 1, 0);
   big_nodes_.clear();
+  big_npanels_.assign(nlev, 0);
+  big_tiles_.assign(nlev, 0);
+  std::vector<int> big_root, big_ns;
+  std::vector<int64_t> big_woff;
+  int64_t wmax = 0;
   for (int l = 0; l < nlev; l++) {
     int mx = 0;
+    int64_t wtot = 0;
     for (int t = level_ptr_[l]; t < level_ptr_[l + 1]; t++) {
       int k = level_nodes_[t];
       if (big[k]) {
         big_nodes_.push_back(k);
-      } else {
-        mx = std::max(mx, std::min(ns[k] + cs[k] + 16, LDS_MAX_M));
+        big_root.push_back(cs[k] == 0 ? 1 : 0);
+        big_ns.push_back(ns[k]);
+        const int mcap = ns[k] + cs[k] + DCAP_BIG;
+        big_woff.push_back(wtot);
+        wtot += (int64_t)mcap * NB;
+        big_npanels_[l] = std::max(big_npanels_[l], (ns[k] + DCAP_BIG) / (NB - 1) + 2);
+        big_tiles_[l] = std::max(big_tiles_[l], (mcap + TILE - 1) / TILE);
       }
+      mx = std::max(mx, std::min(ns[k] + cs[k] + 32, LDS_MAX_M));
     }
     level_lds_[l] = mx * mx;
     big_ptr_[l + 1] = int(big_nodes_.size());
+    wmax = std::max(wmax, wtot);
   }
 
   std::vector<int> parent(plan.node_parent);
@@ -957,6 +1282,12 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   if (upload(allocs_, aidx, &P_.aidx, err)) return -2;
   if (upload(allocs_, arow, &P_.arow, err)) return -2;
   if (upload(allocs_, acol, &P_.acol, err)) return -2;
+  {
+    std::vector<int> rp(rowp, rowp + n + 1), cl(cols, cols + rowp[n]);
+    if (upload(allocs_, rp, &d_rowp_, err)) return -2;
+    if (upload(allocs_, cl, &d_cols_, err)) return -2;
+    if (dalloc(allocs_, std::max(n, 1), &scale_, err)) return -2;
+  }
   P_.n = n;
   P_.K = K;
   const int* dl = nullptr;
@@ -965,6 +1296,25 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   const int* db = nullptr;
   if (upload(allocs_, big, &db, err)) return -2;
   d_big_mask_ = const_cast<int*>(db);
+  {
+    const int* t1 = nullptr;
+    const int* t2 = nullptr;
+    const int* t3 = nullptr;
+    const int64_t* t4 = nullptr;
+    if (upload(allocs_, big_nodes_, &t1, err)) return -2;
+    if (upload(allocs_, big_root, &t2, err)) return -2;
+    if (upload(allocs_, big_ns, &t3, err)) return -2;
+    if (upload(allocs_, big_woff, &t4, err)) return -2;
+    d_big_nodes_ = t1;
+    d_big_root_ = t2;
+    d_big_ns_ = t3;
+    d_big_woff_ = t4;
+    if (dalloc(allocs_, std::max<size_t>(1, (size_t)K), &d_big_woff_sync_, err)) return -2;
+    if (wmax > 0) {
+      HIPCHK(hipMalloc(&W_, wmax * sizeof(double)));
+      W_cap_ = wmax;
+    }
+  }
 
   if (dalloc(allocs_, K, &S_.D, err)) return -2;
   if (dalloc(allocs_, K, &S_.npiv, err)) return -2;
@@ -972,7 +1322,10 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   if (dalloc(allocs_, K, &S_.m, err)) return -2;
   if (dalloc(allocs_, K, &S_.foff, err)) return -2;
   if (dalloc(allocs_, K, &S_.voff, err)) return -2;
-  if (dalloc(allocs_, 4 * (size_t)K, &S_.pstate, err)) return -2;
+  if (dalloc(allocs_, PS * (size_t)K, &S_.pstate, err)) return -2;
+  if (dalloc(allocs_, K, &d_sync_nodes_, err)) return -2;
+  if (dalloc(allocs_, K, &d_sync_root_, err)) return -2;
+  if (dalloc(allocs_, K, &d_sync_ns_, err)) return -2;
   if (dalloc(allocs_, 2, &S_.tops, err)) return -2;
   if (dalloc(allocs_, 8, &S_.flags, err)) return -2;
 
@@ -981,7 +1334,7 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
     if (!attr_done) {
       HIPCHK(hipFuncSetAttribute((const void*)level_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 LDS_MAX_M * LDS_MAX_M * (int)sizeof(double)));
+                                 (LDS_MAX_M * LDS_MAX_M + LDS_MAX_M) * (int)sizeof(double)));
       HIPCHK(hipFuncSetAttribute((const void*)solve_fwd_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  150 * 1024));
@@ -1024,62 +1377,111 @@ int LdlNumeric::ensure_pools(unsigned long long rcap, unsigned long long icap,
 }
 
 int LdlNumeric::run_factor(const double* vals, const double* shift,
-                           FactorParams prm, bool cholesky,
+                           FactorParams prm, bool cholesky, bool sync_mode,
                            hipStream_t stream, std::string& err) {
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, stream, S_);
   HIPCHK(hipGetLastError());
+  if (scaling_) {
+    int nb = (n_ * 64 + NT - 1) / NT;
+    nb = std::min(std::max(nb, 1), 8192);
+    hipLaunchKernelGGL(equilibrate_kernel, dim3(nb), dim3(NT), 0, stream, n_,
+                       d_rowp_, d_cols_, vals, shift, scale_);
+    HIPCHK(hipGetLastError());
+  }
   const int nlev = num_levels();
-  std::vector<int> hm, hD;
   for (int l = 0; l < nlev; l++) {
     const int cnt = level_ptr_[l + 1] - level_ptr_[l];
     const int lds = level_lds_[l];
+    tk_begin(stream);
     hipLaunchKernelGGL(level_kernel, dim3(cnt), dim3(NT),
-                       size_t(lds) * sizeof(double), stream, P_, S_, vals,
+                       size_t(lds + LDS_MAX_M) * sizeof(double), stream, P_, S_, vals,
                        shift, d_level_nodes_ + level_ptr_[l], d_big_mask_, lds,
-                       prm.ustab, prm.pivot_tol, cholesky ? 1 : 0);
+                       prm.ustab, prm.pivot_tol, cholesky ? 1 : 0,
+                       sync_mode ? -1 : DCAP_BIG,
+                       (const double*)(scaling_ ? scale_ : nullptr));
+    tk_end(stream, TK_LEVEL);
     HIPCHK(hipGetLastError());
-    if (big_ptr_[l + 1] == big_ptr_[l]) continue;
-    // Big fronts: sizes are known only after assembly (delayed pivots).
-    HIPCHK(hipStreamSynchronize(stream));
-    int flag = 0;
-    HIPCHK(hipMemcpy(&flag, S_.flags, sizeof(int), hipMemcpyDeviceToHost));
-    if (flag != 0) return flag;
-    for (int t = big_ptr_[l]; t < big_ptr_[l + 1]; t++) {
-      const int node = big_nodes_[t];
-      int mD[1], Dv[1];
-      HIPCHK(hipMemcpy(mD, S_.m + node, sizeof(int), hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(Dv, S_.D + node, sizeof(int), hipMemcpyDeviceToHost));
-      const int m = mD[0];
-      const int nfs = h_ns_[node] + Dv[0];
-      const size_t wneed = (size_t)m * NB;
-      if (wneed > W_cap_) {
+    const int b0 = big_ptr_[l], nb = big_ptr_[l + 1] - big_ptr_[l];
+    if (nb == 0 && !sync_mode) continue;
+    int npanels = big_npanels_[l], tiles = big_tiles_[l];
+    const int* bnodes = d_big_nodes_ + b0;
+    const int* broot = d_big_root_ + b0;
+    const int* bns = d_big_ns_ + b0;
+    const int64_t* woff = d_big_woff_ + b0;
+    int nlaunch = nb;
+    if (sync_mode) {
+      // Front sizes are known only after assembly: read them back and list
+      // the fronts of this level that exceed the workgroup limit.
+      HIPCHK(hipStreamSynchronize(stream));
+      int flag = 0;
+      HIPCHK(hipMemcpy(&flag, S_.flags, sizeof(int), hipMemcpyDeviceToHost));
+      if (flag != 0) return flag;
+      std::vector<int> hm(K_), hD(K_);
+      HIPCHK(hipMemcpy(hm.data(), S_.m, K_ * sizeof(int), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(hD.data(), S_.D, K_ * sizeof(int), hipMemcpyDeviceToHost));
+      std::vector<int> ln, lr, lns;
+      std::vector<int64_t> wo;
+      int64_t tot = 0;
+      npanels = 0;
+      tiles = 0;
+      for (int t = level_ptr_[l]; t < level_ptr_[l + 1]; t++) {
+        const int node = level_nodes_[t];
+        if (hm[node] <= LDS_MAX_M) continue;
+        ln.push_back(node);
+        lr.push_back(h_cs_[node] == 0 ? 1 : 0);
+        lns.push_back(h_ns_[node]);
+        wo.push_back(tot);
+        tot += (int64_t)hm[node] * NB;
+        npanels = std::max(npanels, (h_ns_[node] + hD[node]) / (NB - 1) + 2);
+        tiles = std::max(tiles, (hm[node] + TILE - 1) / TILE);
+      }
+      nlaunch = int(ln.size());
+      if (nlaunch == 0) continue;
+      if ((size_t)tot > W_cap_) {
         if (W_) HIPCHK(hipFree(W_));
         W_ = nullptr;
-        HIPCHK(hipMalloc(&W_, wneed * sizeof(double)));
-        W_cap_ = wneed;
+        HIPCHK(hipMalloc(&W_, tot * sizeof(double)));
+        W_cap_ = tot;
       }
-      const int root = h_cs_[node] == 0 ? 1 : 0;
-      const int npanels = nfs / (NB - 1) + 2;
-      const int tiles = (m + TILE - 1) / TILE;
-      for (int q = 0; q < npanels; q++) {
-        hipLaunchKernelGGL(panel_kernel, dim3(1), dim3(NT), 0, stream, S_,
-                           node, W_, prm.ustab, prm.pivot_tol, root,
-                           cholesky ? 1 : 0);
-        hipLaunchKernelGGL(trailing_kernel, dim3(tiles, tiles), dim3(NT), 0,
-                           stream, S_, node, (const double*)W_);
-      }
-      hipLaunchKernelGGL(finalize_big_kernel, dim3(1), dim3(1), 0, stream, S_,
-                         node, nfs);
-      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpy(d_big_woff_sync_, wo.data(), nlaunch * sizeof(int64_t),
+                       hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d_sync_nodes_, ln.data(), nlaunch * sizeof(int), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d_sync_root_, lr.data(), nlaunch * sizeof(int), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d_sync_ns_, lns.data(), nlaunch * sizeof(int), hipMemcpyHostToDevice));
+      bnodes = d_sync_nodes_;
+      broot = d_sync_root_;
+      bns = d_sync_ns_;
+      woff = d_big_woff_sync_;
     }
+    for (int q = 0; q < npanels; q++) {
+      tk_begin(stream);
+      hipLaunchKernelGGL(panel_kernel, dim3(nlaunch), dim3(NT), 0, stream, S_,
+                         bnodes, broot, W_, woff, prm.ustab, prm.pivot_tol,
+                         cholesky ? 1 : 0);
+      tk_end(stream, TK_PANEL);
+      tk_begin(stream);
+      hipLaunchKernelGGL(trailing_kernel, dim3(tiles, tiles, nlaunch), dim3(NT), 0,
+                         stream, S_, bnodes, (const double*)W_, woff);
+      tk_end(stream, TK_TRAILING);
+    }
+    hipLaunchKernelGGL(finalize_big_kernel, dim3((nlaunch + 63) / 64), dim3(64), 0,
+                       stream, S_, bnodes, bns, nlaunch);
+    HIPCHK(hipGetLastError());
   }
   HIPCHK(hipStreamSynchronize(stream));
+  if (timing_) tk_collect();
   int flags[8];
   HIPCHK(hipMemcpy(flags, S_.flags, sizeof(flags), hipMemcpyDeviceToHost));
   if (flags[0] != 0) return flags[0];
+  if (flags[4] != 0) {
+    err = "non-finite pivot (the matrix holds NaN or Inf)";
+    return -6;
+  }
   inertia_[0] = flags[1];
   inertia_[1] = flags[2];
   inertia_[2] = flags[3];
+  delayed_ = flags[5];
+  max_front_seen_ = flags[6];
   return 0;
 }
 
@@ -1088,8 +1490,15 @@ int LdlNumeric::factor(const double* vals, const double* shift, double ustab,
                        std::string& err) {
   FactorParams prm{ustab, pivot_tol};
   factored_ = false;
-  for (int attempt = 0; attempt < 6; attempt++) {
-    int rc = run_factor(vals, shift, prm, cholesky, stream, err);
+  bool sync_mode = false;
+  for (int attempt = 0; attempt < 8; attempt++) {
+    int rc = run_factor(vals, shift, prm, cholesky, sync_mode, stream, err);
+    if (rc == -7) {
+      // More delayed pivots than the static launch plan holds: redo the
+      // factorization with per-level front sizes.
+      sync_mode = true;
+      continue;
+    }
     if (rc == -4) {
       // Delayed pivots outgrew the pools: grow and refactor.
       unsigned long long used[2];
@@ -1128,17 +1537,32 @@ int LdlNumeric::solve(int nrhs, double* x, int ldx, hipStream_t stream,
     return -1;
   }
   const int nlev = num_levels();
+  const int sb = std::min(4096, std::max(1, (n_ * std::max(nrhs, 1) + NT - 1) / NT));
+  if (scaling_)
+    hipLaunchKernelGGL(scale_vec_kernel, dim3(sb), dim3(NT), 0, stream, n_, nrhs, ldx,
+                       (const double*)scale_, x);
   for (int l = 0; l < nlev; l++) {
     const int cnt = level_ptr_[l + 1] - level_ptr_[l];
+    tk_begin(stream);
     hipLaunchKernelGGL(solve_fwd_kernel, dim3(cnt), dim3(NT), lds, stream, S_,
                        d_level_nodes_ + level_ptr_[l], x, ldx, nrhs);
+    tk_end(stream, TK_SOLVE_FWD);
   }
   for (int l = nlev - 1; l >= 0; l--) {
     const int cnt = level_ptr_[l + 1] - level_ptr_[l];
+    tk_begin(stream);
     hipLaunchKernelGGL(solve_bwd_kernel, dim3(cnt), dim3(NT), lds, stream, S_,
                        d_level_nodes_ + level_ptr_[l], x, ldx, nrhs);
+    tk_end(stream, TK_SOLVE_BWD);
   }
+  if (scaling_)
+    hipLaunchKernelGGL(scale_vec_kernel, dim3(sb), dim3(NT), 0, stream, n_, nrhs, ldx,
+                       (const double*)scale_, x);
   HIPCHK(hipGetLastError());
+  if (timing_) {
+    HIPCHK(hipStreamSynchronize(stream));
+    tk_collect();
+  }
   return 0;
 }
 

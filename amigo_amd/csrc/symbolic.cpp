@@ -548,7 +548,42 @@ void dissect(NDWork& w, std::vector<int> verts, int* out, int leaf) {
 
 }  // namespace
 
-void nested_dissection(int n, const int* rowp, const int* cols, int* perm) {
+void nested_dissection(int n, const int* rowp, const int* cols, int* perm,
+                       const int* groups) {
+  if (groups != nullptr) {
+    // Dissect the quotient graph of the variable groups (e.g. the grid or
+    // time nodes of a model), then list each group's variables in increasing
+    // index order.  A multiplier laid out after its primal partners is then
+    // eliminated in the same front as them, where a 2x2 pivot is available.
+    int G = 0;
+    for (int i = 0; i < n; i++) G = std::max(G, groups[i] + 1);
+    std::vector<int> gp(G + 1, 0), gcols;
+    std::vector<std::vector<int>> adj(G);
+    for (int i = 0; i < n; i++)
+      for (int p = rowp[i]; p < rowp[i + 1]; p++) {
+        int gi = groups[i], gj = groups[cols[p]];
+        if (gi != gj) adj[gi].push_back(gj);
+      }
+    for (int g = 0; g < G; g++) {
+      std::sort(adj[g].begin(), adj[g].end());
+      adj[g].erase(std::unique(adj[g].begin(), adj[g].end()), adj[g].end());
+      gp[g + 1] = gp[g] + int(adj[g].size());
+      gcols.insert(gcols.end(), adj[g].begin(), adj[g].end());
+    }
+    std::vector<int> gperm(G);
+    nested_dissection(G, gp.data(), gcols.data(), gperm.data(), nullptr);
+    std::vector<int> mptr(G + 1, 0), members(n);
+    for (int i = 0; i < n; i++) mptr[groups[i] + 1]++;
+    for (int g = 0; g < G; g++) mptr[g + 1] += mptr[g];
+    std::vector<int> fill(mptr.begin(), mptr.end() - 1);
+    for (int i = 0; i < n; i++) members[fill[groups[i]]++] = i;
+    int k = 0;
+    for (int t = 0; t < G; t++) {
+      int g = gperm[t];
+      for (int q = mptr[g]; q < mptr[g + 1]; q++) perm[k++] = members[q];
+    }
+    return;
+  }
   NDWork w;
   w.n = n;
   w.rowp = rowp;
@@ -562,7 +597,8 @@ void nested_dissection(int n, const int* rowp, const int* cols, int* perm) {
 }
 
 void compute_ordering(int order, int n, const int* rowp, const int* cols,
-                      std::vector<int>& perm, std::vector<int>& iperm) {
+                      std::vector<int>& perm, std::vector<int>& iperm,
+                      const int* groups) {
   perm.assign(n, 0);
   iperm.assign(n, 0);
   if (order == ORDER_NATURAL || order == ORDER_MULTI_COLOR) {
@@ -573,7 +609,7 @@ void compute_ordering(int order, int n, const int* rowp, const int* cols,
     if (order == ORDER_AMD) {
       block_amd(n, rp.data(), cl.data(), 0, nullptr, perm.data());
     } else {
-      nested_dissection(n, rp.data(), cl.data(), perm.data());
+      nested_dissection(n, rp.data(), cl.data(), perm.data(), groups);
     }
   }
   for (int i = 0; i < n; i++) iperm[perm[i]] = i;
@@ -827,13 +863,13 @@ void contribution_rows(int n, const int* colp, const int* rows,
 
 // SparseLDL::symbolic_analysis (sparse_ldl.h:2204-2358).
 void symbolic_analysis(int order, int n, const int* rowp, const int* cols,
-                       SymbolicPlan& P) {
+                       SymbolicPlan& P, const int* groups) {
   P.n = n;
   P.order = order;
   const int* perm = nullptr;
   const int* iperm = nullptr;
   if (order != ORDER_NATURAL) {
-    compute_ordering(order, n, rowp, cols, P.perm, P.iperm);
+    compute_ordering(order, n, rowp, cols, P.perm, P.iperm, groups);
     perm = P.perm.data();
     iperm = P.iperm.data();
   } else {

@@ -31,12 +31,16 @@ void copy_for_reorder(int order, int n, const int* rowp, const int* cols,
                       std::vector<int>& rowp_out, std::vector<int>& cols_out);
 
 // Nested dissection by recursive graph bisection (this build's own ND: the
-// reference's ND is METIS_NodeND, or the identity without METIS).
-void nested_dissection(int n, const int* rowp, const int* cols, int* perm);
+// reference's ND is METIS_NodeND, or the identity without METIS).  With
+// `groups` (a group id per variable) the quotient graph of the groups is
+// dissected and each group's variables stay together in index order.
+void nested_dissection(int n, const int* rowp, const int* cols, int* perm,
+                       const int* groups = nullptr);
 
 // perm/iperm as SparseLDL computes them (ordering_utils.h:38-56).
 void compute_ordering(int order, int n, const int* rowp, const int* cols,
-                      std::vector<int>& perm, std::vector<int>& iperm);
+                      std::vector<int>& perm, std::vector<int>& iperm,
+                      const int* groups = nullptr);
 
 // Elimination tree, postorder and column counts in the permuted ordering
 // (sparse_ldl.h:2375-2518, 2552-2701).  perm/iperm may be null (identity).
@@ -70,6 +74,6 @@ struct SymbolicPlan {
 // Full symbolic analysis as in SparseLDL's constructor.  The CSR must hold
 // the full symmetric pattern with sorted rows.
 void symbolic_analysis(int order, int n, const int* rowp, const int* cols,
-                       SymbolicPlan& plan);
+                       SymbolicPlan& plan, const int* groups = nullptr);
 
 }  // namespace amigo_amd

@@ -118,14 +118,20 @@ class SparseLDL:
         delay_growth=2.0,
         order="NATURAL",
         precision="fp64",
+        groups=None,
+        scaling=False,
     ):
         self.mat = mat
         self.n = mat.nrows
         self.solver_type = CHOLESKY if str(solver_type).upper().endswith("CHOLESKY") else LDL
         self.precision = precision
         h = ctypes.c_void_p()
+        gp = None
+        if groups is not None:
+            self._groups = np.ascontiguousarray(groups, dtype=np.int32)
+            gp = _lib.ip(self._groups)
         _lib.check(
-            _lib.lib().amigo_ldl_create(
+            _lib.lib().amigo_ldl_create_grouped(
                 self.n,
                 _lib.ip(mat.rowp),
                 _lib.ip(mat.cols),
@@ -134,11 +140,14 @@ class SparseLDL:
                 float(pivot_tol),
                 float(delay_growth),
                 _order_code(order),
+                gp,
                 ctypes.byref(h),
             )
         )
         self._h = h
         self._factored = False
+        if scaling:
+            _lib.check(_lib.lib().amigo_ldl_set_scaling(self._h, 1))
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -230,6 +239,24 @@ class SparseLDL:
             )
         )
         return x
+
+    def stats(self):
+        """(delayed pivots, largest front) of the last factorization."""
+        d, m = ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.lib().amigo_ldl_get_stats(self._h, ctypes.byref(d), ctypes.byref(m)))
+        return d.value, m.value
+
+    def set_timing(self, on=True, reset=False):
+        _lib.check(_lib.lib().amigo_ldl_set_timing(self._h, 2 if (on and reset) else int(bool(on))))
+
+    def timing(self):
+        """{class: (ms, launches)} and the flops split (small, big)."""
+        ms = (ctypes.c_double * 5)()
+        cnt = (ctypes.c_longlong * 5)()
+        fs, fb = ctypes.c_double(), ctypes.c_double()
+        _lib.check(_lib.lib().amigo_ldl_get_timing(self._h, ms, cnt, ctypes.byref(fs), ctypes.byref(fb)))
+        names = ["level", "panel", "schur_update", "solve_fwd", "solve_bwd"]
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(names)}, (fs.value, fb.value)
 
     def work(self):
         f, b = ctypes.c_double(), ctypes.c_double()

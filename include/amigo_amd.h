@@ -24,6 +24,7 @@ enum {
   AMIGO_ERR_STATE = -3,      /* call out of order (e.g. solve before factor) */
   AMIGO_ERR_CAPACITY = -4,   /* delayed pivots exceeded the front pool */
   AMIGO_ERR_NOMEM = -5,      /* device allocation failed */
+  AMIGO_ERR_NUMERIC = -6,    /* non-finite pivot during factorization */
 };
 
 /* Orderings, numbered as include/ordering_utils.h:21 OrderingType. */
@@ -72,6 +73,15 @@ typedef struct amigo_ldl amigo_ldl;
 int amigo_ldl_create(int n, const int* rowp, const int* cols, int solver_type,
                      double ustab, double pivot_tol, double delay_growth,
                      int order, amigo_ldl** out);
+/* Same, with a group id per variable (e.g. the model node it belongs to;
+ * NULL = none).  Used by the nested-dissection ordering only: the quotient
+ * graph of the groups is dissected and each group's variables are kept
+ * together in index order.  No reference counterpart (the reference's ND is
+ * METIS on the variable graph). */
+int amigo_ldl_create_grouped(int n, const int* rowp, const int* cols,
+                             int solver_type, double ustab, double pivot_tol,
+                             double delay_growth, int order,
+                             const int* var_group, amigo_ldl** out);
 int amigo_ldl_destroy(amigo_ldl* h);
 
 /* Sizes of the symbolic arrays: num_nodes, total contribution rows, number
@@ -111,6 +121,25 @@ int amigo_ldl_get_inertia(const amigo_ldl* h, int* npos, int* nneg,
  * (MatrixFactor::get_factor, include/sparse_ldl.h:411-445). */
 int amigo_ldl_pivot_counts(const amigo_ldl* h, int* num_pivots,
                            int* num_delayed);
+
+/* Symmetric equilibration before each factorization: mode 0 = none (the
+ * reference's behaviour, default), 1 = scale row/column i by
+ * 1/sqrt(max_j |a_ij|).  Inertia is unchanged; solves unscale.  No
+ * reference counterpart. */
+int amigo_ldl_set_scaling(amigo_ldl* h, int mode);
+
+/* HIP-event timing of every factor/solve launch (on = 1; 2 also resets
+ * the totals), and the accumulated milliseconds / launch counts per kernel
+ * class: [level (assemble + workgroup factor), panel, Schur update,
+ * forward solve, backward solve], plus the algorithmic flops of the fronts
+ * each factorization path owns.  No reference counterpart. */
+int amigo_ldl_set_timing(amigo_ldl* h, int on);
+int amigo_ldl_get_timing(const amigo_ldl* h, double* ms5, long long* count5,
+                         double* flops_small, double* flops_big);
+
+/* Total delayed pivots and largest front of the last factorization (the
+ * reference keeps these per node in MatrixFactor, sparse_ldl.h:477-489). */
+int amigo_ldl_get_stats(const amigo_ldl* h, int* num_delayed, int* max_front);
 
 /* SparseLDL::solve(nrhs, x, ldx) (include/sparse_ldl.h:184-190): x is a
  * device column-major n x nrhs block, overwritten in place. */
