@@ -553,8 +553,12 @@ void nested_dissection(int n, const int* rowp, const int* cols, int* perm,
   if (groups != nullptr) {
     // Dissect the quotient graph of the variable groups (e.g. the grid or
     // time nodes of a model), then list each group's variables in increasing
-    // index order.  A multiplier laid out after its primal partners is then
-    // eliminated in the same front as them, where a 2x2 pivot is available.
+    // index order, those coupled only to their own group last.  A multiplier
+    // laid out after its primal partners is then eliminated in the same front
+    // as them, where a 2x2 pivot is available; a group-internal variable
+    // (a control coupled to its own constraint only) placed last is filled
+    // into the clique of the earlier columns, so consecutive groups of a
+    // separator chain into one supernode instead of one node per group.
     int G = 0;
     for (int i = 0; i < n; i++) G = std::max(G, groups[i] + 1);
     std::vector<int> gp(G + 1, 0), gcols;
@@ -577,10 +581,16 @@ void nested_dissection(int n, const int* rowp, const int* cols, int* perm,
     for (int g = 0; g < G; g++) mptr[g + 1] += mptr[g];
     std::vector<int> fill(mptr.begin(), mptr.end() - 1);
     for (int i = 0; i < n; i++) members[fill[groups[i]]++] = i;
+    std::vector<char> internal(n, 1);
+    for (int i = 0; i < n; i++)
+      for (int p = rowp[i]; p < rowp[i + 1]; p++)
+        if (groups[cols[p]] != groups[i]) internal[i] = 0;
     int k = 0;
     for (int t = 0; t < G; t++) {
       int g = gperm[t];
-      for (int q = mptr[g]; q < mptr[g + 1]; q++) perm[k++] = members[q];
+      for (int pass = 0; pass < 2; pass++)
+        for (int q = mptr[g]; q < mptr[g + 1]; q++)
+          if (internal[members[q]] == pass) perm[k++] = members[q];
     }
     return;
   }
