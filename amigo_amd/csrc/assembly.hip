@@ -126,17 +126,19 @@ __device__ void eval_elliptic(const double* z, const double* d, double alpha, do
   H[(NB + 2) * NC + NB + 1] = H[(NB + 1) * NC + NB + 2] = -h2;
 }
 
-// Cart-pole accelerations (point-mass pole of length l on a cart):
+// Cart-pole accelerations, point-mass pole of length l on a cart, angle th
+// measured from hanging (th = 0 down, th = pi upright), the convention of the
+// reference's examples/trajectory/cart/cart_pole.py:76-86:
 //   D = mc + mp sin^2(th)
-//   xdd = (u + mp sin(th) (l w^2 - g cos(th))) / D
-//   thdd = (-u cos(th) - mp l w^2 sin(th) cos(th) + (mc + mp) g sin(th)) / (l D)
+//   xdd = (u + mp sin(th) (l w^2 + g cos(th))) / D
+//   thdd = (-u cos(th) - mp l w^2 sin(th) cos(th) - (mc + mp) g sin(th)) / (l D)
 template <class T>
 __device__ void cp_accel(T th, T w, T u, double mc, double mp, double l, T& xdd, T& thdd) {
   const double gr = 9.81;
   T s = hsin(th), c = hcos(th);
   T D = hd(mc) + mp * (s * s);
-  xdd = (u + mp * (s * (l * (w * w) - gr * c))) / D;
-  thdd = ((-1.0) * (u * c) - (mp * l) * ((w * w) * (s * c)) + ((mc + mp) * gr) * s) / (l * D);
+  xdd = (u + mp * (s * (l * (w * w) + gr * c))) / D;
+  thdd = ((-1.0) * (u * c) - (mp * l) * ((w * w) * (s * c)) - ((mc + mp) * gr) * s) / (l * D);
 }
 
 // AMIGO_ELEM_CARTPOLE: trapezoidal collocation interval.

@@ -220,10 +220,13 @@ def elliptic(z, h, yd, a, alpha, dim):
 
 
 def cartpole_accel(th, w, u, mc, mp, l, g=9.81):
+    """Cart-pole accelerations with th measured from hanging: the residual
+    form of examples/trajectory/cart/cart_pole.py:76-86 solved for
+    (xdd, thdd)."""
     s, c = np.sin(th), np.cos(th)
     D = mc + mp * s * s
-    xdd = (u + mp * s * (l * w * w - g * c)) / D
-    thdd = (-u * c - mp * l * w * w * s * c + (mc + mp) * g * s) / (l * D)
+    xdd = (u + mp * s * (l * w * w + g * c)) / D
+    thdd = (-u * c - mp * l * w * w * s * c - (mc + mp) * g * s) / (l * D)
     return xdd, thdd
 
 
@@ -252,3 +255,41 @@ def fd_grad_hess(fun, z, eps=1e-5):
             f[j] = eps
             H[i, j] = (fun(z + e + f) - fun(z + e - f) - fun(z - e + f) + fun(z - e - f)) / (4 * eps * eps)
     return g, H
+
+
+def di_discrete_optimum(N, T=1.0):
+    """Exact optimum of the double-integrator collocation QP (bounds
+    inactive): one dense KKT solve of min sum (u_k^2 + u_{k+1}^2) dt/4
+    s.t. the trapezoidal defects and q(0)=v(0)=0, q(T)=1, v(T)=0.
+    Returns (q, v, u) at the N+1 nodes and the defect multipliers (lq, lv)."""
+    dt = T / N
+    M = N + 1
+    nz = 3 * M                       # z = [q_0..q_N, v_0..v_N, u_0..u_N]
+    iq, iv, iu = np.arange(M), M + np.arange(M), 2 * M + np.arange(M)
+    H = np.zeros((nz, nz))
+    for k in range(N):
+        H[iu[k], iu[k]] += dt / 2
+        H[iu[k + 1], iu[k + 1]] += dt / 2
+    rows = []
+    rhs = []
+    for k in range(N):
+        r = np.zeros(nz)
+        r[iq[k + 1]], r[iq[k]], r[iv[k]], r[iv[k + 1]] = 1.0, -1.0, -dt / 2, -dt / 2
+        rows.append(r)
+        rhs.append(0.0)
+    for k in range(N):
+        r = np.zeros(nz)
+        r[iv[k + 1]], r[iv[k]], r[iu[k]], r[iu[k + 1]] = 1.0, -1.0, -dt / 2, -dt / 2
+        rows.append(r)
+        rhs.append(0.0)
+    for idx, val in ((iq[0], 0.0), (iv[0], 0.0), (iq[N], 1.0), (iv[N], 0.0)):
+        r = np.zeros(nz)
+        r[idx] = 1.0
+        rows.append(r)
+        rhs.append(val)
+    A = np.array(rows)
+    m = A.shape[0]
+    K = np.block([[H, A.T], [A, np.zeros((m, m))]])
+    sol = np.linalg.solve(K, np.concatenate([np.zeros(nz), rhs]))
+    z, lam = sol[:nz], sol[nz:]
+    return z[iq], z[iv], z[iu], lam[:N], lam[N:2 * N]
