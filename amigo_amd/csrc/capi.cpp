@@ -284,10 +284,10 @@ int amigo_ldl_set_timing(amigo_ldl* h, int on) {
   return AMIGO_OK;
 }
 
-int amigo_ldl_get_timing(const amigo_ldl* h, double* ms5, long long* count5,
+int amigo_ldl_get_timing(const amigo_ldl* h, double* ms6, long long* count6,
                          double* flops_small, double* flops_big) {
-  if (!h || !ms5 || !count5) return fail(AMIGO_ERR_ARG, "null argument");
-  h->num.get_timing(ms5, count5);
+  if (!h || !ms6 || !count6) return fail(AMIGO_ERR_ARG, "null argument");
+  h->num.get_timing(ms6, count6);
   if (flops_small) *flops_small = h->num.flops_small();
   if (flops_big) *flops_big = h->num.flops_big();
   return AMIGO_OK;

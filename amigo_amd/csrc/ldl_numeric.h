@@ -41,6 +41,7 @@ struct DevState {
   int* pstate = nullptr;   // [K*8] blocked path: k, nc, kstart, kw, blocked
   double* rpool = nullptr;
   int* ipool = nullptr;
+  int* fpos = nullptr;     // [icap] final front position of each assembly position
   unsigned long long* tops = nullptr;  // [2] bump pointers (rpool, ipool)
   unsigned long long rcap = 0, icap = 0;
   int* flags = nullptr;    // [8]: 0 error, 1..3 inertia pos/neg/zero
@@ -52,7 +53,15 @@ struct FactorParams {
 };
 
 // Kernel classes timed with HIP events when timing is on.
-enum TimedKernel { TK_LEVEL = 0, TK_PANEL = 1, TK_TRAILING = 2, TK_SOLVE_FWD = 3, TK_SOLVE_BWD = 4, TK_COUNT = 5 };
+enum TimedKernel {
+  TK_LEVEL = 0,
+  TK_PANEL = 1,
+  TK_TRAILING = 2,
+  TK_SOLVE_FWD = 3,
+  TK_SOLVE_BWD = 4,
+  TK_ASSEMBLE = 5,
+  TK_COUNT = 6
+};
 
 class LdlNumeric {
  public:
@@ -120,6 +129,9 @@ class LdlNumeric {
   std::vector<int> big_nodes_;      // per level list of big fronts
   std::vector<int> big_ptr_;
   std::vector<int> big_npanels_, big_tiles_;  // static launch plan per level
+  std::vector<int> big_maxkids_;    // most children of a blocked front, per level
+  std::vector<int> h_kptr_;         // children pointer (host copy)
+  std::vector<int> h_m_;            // front sizes read back in sync mode
   const int* d_big_nodes_ = nullptr;
   const int* d_big_root_ = nullptr;
   const int* d_big_ns_ = nullptr;
@@ -135,6 +147,8 @@ class LdlNumeric {
   std::vector<int> h_ns_, h_cs_;
   int max_front_ = 0;
   double* W_ = nullptr;             // panel workspace (max_m x 64)
+  double* ub_ = nullptr;            // forward-solve update vectors (icap x nrhs)
+  size_t ub_cap_ = 0;
   size_t W_cap_ = 0;
   double delay_growth_ = 2.0;
   unsigned long long base_rcap_ = 0, base_icap_ = 0;
@@ -144,8 +158,8 @@ class LdlNumeric {
   int max_p_ = 0;                   // largest fully summed block seen
   bool scaling_ = false;
   bool timing_ = false;
-  double tk_ms_[TK_COUNT] = {0, 0, 0, 0, 0};
-  long long tk_n_[TK_COUNT] = {0, 0, 0, 0, 0};
+  double tk_ms_[TK_COUNT] = {0, 0, 0, 0, 0, 0};
+  long long tk_n_[TK_COUNT] = {0, 0, 0, 0, 0, 0};
   std::vector<hipEvent_t> ev_;
   std::vector<int> ev_class_;
   int ev_used_ = 0;

@@ -549,6 +549,17 @@ __global__ __launch_bounds__(NT) void level_kernel(
       off += nd;
     }
   }
+  if (big) {
+    // Assembled by the grid-wide kernels of the blocked path.
+    if (tid == 0) {
+      S.pstate[PS * node + 0] = 0;
+      S.pstate[PS * node + 1] = ns + D;
+      S.pstate[PS * node + 2] = 0;
+      S.pstate[PS * node + 3] = 0;
+      S.pstate[PS * node + 4] = 1;
+    }
+    return;
+  }
   const size_t mm = (size_t)m * m;
   for (size_t e = tid; e < mm; e += NT) F[e] = 0.0;
   __syncthreads();
@@ -599,17 +610,6 @@ __global__ __launch_bounds__(NT) void level_kernel(
     }
   }
 
-  if (big) {
-    if (tid == 0) {
-      S.pstate[PS * node + 0] = 0;
-      S.pstate[PS * node + 1] = ns + D;
-      S.pstate[PS * node + 2] = 0;
-      S.pstate[PS * node + 3] = 0;
-      S.pstate[PS * node + 4] = 1;
-    }
-    return;
-  }
-
   int np;
   if (in_lds) {
     np = factor_front_lds(F, ld, m, ns + D, vars, cs == 0, chol != 0, ustab,
@@ -637,6 +637,89 @@ __global__ __launch_bounds__(NT) void level_kernel(
     if (cnt[2]) atomicAdd(&S.flags[3], cnt[2]);
     if (cnt[3]) atomicAdd(&S.flags[4], cnt[3]);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Grid-wide assembly of the blocked-path fronts of a level (the level kernel
+// only allocated them).  Three stream-ordered steps, each free of write
+// conflicts, so the sums are deterministic: (1) zero the front and place the
+// diagonal shift of the supernode pivots; (2) add the original entries;
+// (3) extend-add the children, one child index per launch (child order as in
+// sparse_ldl.h:1084-1152).
+__device__ __forceinline__ bool big_active(const DevState& S, int node) {
+  return S.pstate[PS * node + 4] == 1 && atomicAdd(&S.flags[0], 0) == 0;
+}
+
+__global__ __launch_bounds__(NT) void big_zero_kernel(DevPlan P, DevState S,
+                                                      const int* __restrict__ bnodes,
+                                                      const double* __restrict__ shift,
+                                                      const double* __restrict__ sc) {
+  const int node = bnodes[blockIdx.y];
+  if (!big_active(S, node)) return;
+  const int m = S.m[node], ns = P.ns[node];
+  double* F = S.rpool + S.foff[node];
+  const int* vars = S.ipool + S.voff[node];
+  for (int j = blockIdx.x; j < m; j += gridDim.x) {
+    double* Fj = F + (size_t)j * m;
+    double dj = 0.0;
+    if (j < ns && shift != nullptr) {
+      const int v = vars[j];
+      dj = sc ? shift[v] * sc[v] * sc[v] : shift[v];
+    }
+    for (int i = threadIdx.x; i < m; i += NT) Fj[i] = i == j ? dj : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(NT) void big_orig_kernel(DevPlan P, DevState S,
+                                                      const int* __restrict__ bnodes,
+                                                      const double* __restrict__ vals,
+                                                      const double* __restrict__ sc) {
+  const int node = bnodes[blockIdx.y];
+  if (!big_active(S, node)) return;
+  const int m = S.m[node], ns = P.ns[node], D = S.D[node];
+  double* F = S.rpool + S.foff[node];
+  const int* vars = S.ipool + S.voff[node];
+  for (int t = P.aptr[node] + blockIdx.x * NT + threadIdx.x; t < P.aptr[node + 1];
+       t += gridDim.x * NT) {
+    int r = P.arow[t];
+    r = r < ns ? r : r + D;
+    const int c = P.acol[t];
+    double v = vals[P.aidx[t]];
+    if (sc) v *= sc[vars[r]] * sc[vars[c]];
+    if (r >= c)
+      F[r + (size_t)c * m] += v;
+    else
+      F[c + (size_t)r * m] += v;
+  }
+}
+
+__global__ __launch_bounds__(NT) void big_child_kernel(DevPlan P, DevState S,
+                                                       const int* __restrict__ bnodes,
+                                                       int q) {
+  const int node = bnodes[blockIdx.y];
+  const int kb = P.kptr[node], ke = P.kptr[node + 1];
+  if (kb + q >= ke || !big_active(S, node)) return;
+  const int m = S.m[node], ns = P.ns[node], D = S.D[node];
+  double* F = S.rpool + S.foff[node];
+  int doff = ns;
+  for (int t = kb; t < kb + q; t++) doff += S.ndel[P.kids[t]];
+  const int c = P.kids[kb + q];
+  const int pc = S.npiv[c], dc = S.ndel[c], mc = S.m[c];
+  const int w = mc - pc;
+  const double* Cf = S.rpool + S.foff[c];
+  for (int jj = blockIdx.x; jj < w; jj += gridDim.x) {
+    const int pj = child_row_pos(P, c, dc, doff, ns, D, jj);
+    const double* Cj = Cf + pc + (size_t)(pc + jj) * mc;
+    for (int ii = jj + threadIdx.x; ii < w; ii += NT) {
+      const int pi = child_row_pos(P, c, dc, doff, ns, D, ii);
+      const double v = Cj[ii];
+      if (pi >= pj)
+        F[pi + (size_t)pj * m] += v;
+      else
+        F[pj + (size_t)pi * m] += v;
+    }
+  }
+  (void)m;
 }
 
 // ---------------------------------------------------------------------------
@@ -940,26 +1023,84 @@ __global__ void reset_kernel(DevState S) {
 }
 
 // ---------------------------------------------------------------------------
-// Level-scheduled solves (sparse_ldl.h:1807-1908).  Forward: t = x[piv];
-// unit-lower solve; x[rows below] -= L21 t; t = D^-1 t; x[piv] = t.
-__global__ __launch_bounds__(NT) void solve_fwd_kernel(DevState S,
+// Pivoting permutes the fully summed rows of a front; the forward solve maps
+// the children's update rows (laid out in assembly order: supernode
+// variables, then the children's delayed pivots in child order) to their
+// final positions through fpos.  One workgroup per front.
+__global__ __launch_bounds__(NT) void fpos_kernel(DevPlan P, DevState S, int K) {
+  // After a failed (to be redone) factorization the fronts are not all
+  // allocated: nothing to map.
+  if (atomicAdd(&S.flags[0], 0) != 0) return;
+  for (int node = blockIdx.x; node < K; node += gridDim.x) {
+    const int ns = P.ns[node], D = S.D[node], nfs = ns + D;
+    const int* vars = S.ipool + S.voff[node];
+    int* fpos = S.fpos + S.voff[node];
+    const int* sv = P.nvars + P.nptr[node];
+    const int kb = P.kptr[node], ke = P.kptr[node + 1];
+    for (int j = threadIdx.x; j < nfs; j += NT) {
+      int v = vars[j];
+      v = v >= 0 ? v : -v - 1;
+      int a = -1;
+      for (int q = 0; q < ns && a < 0; q++)
+        if (sv[q] == v) a = q;
+      int doff = ns;
+      for (int t = kb; t < ke && a < 0; t++) {
+        const int c = P.kids[t];
+        const int pc = S.npiv[c], dc = S.ndel[c];
+        const int* cv = S.ipool + S.voff[c] + pc;
+        for (int q = 0; q < dc && a < 0; q++)
+          if (cv[q] == v) a = doff + q;
+        doff += dc;
+      }
+      if (a >= 0) fpos[a] = j;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Level-scheduled solves (sparse_ldl.h:1807-1908).  Forward, multifrontal
+// form: the front vector t holds the right-hand side on the pivot rows plus
+// the children's update vectors (extend-added in child order, so the sums
+// are deterministic and no atomics are needed); the unit-lower solve on the
+// pivots follows, the front's own update vector u = t[p:m] - L21 t[0:p] is
+// stored for the parent, then t = D^-1 t goes to x.  Update vectors live in
+// ub at the front's index-pool offset (ld of one right-hand side = ubld).
+__global__ __launch_bounds__(NT) void solve_fwd_kernel(DevPlan P, DevState S,
                                                        const int* __restrict__ lnodes,
                                                        double* __restrict__ x,
-                                                       int ldx, int nrhs) {
+                                                       int ldx, int nrhs,
+                                                       double* __restrict__ ub,
+                                                       size_t ubld) {
   extern __shared__ double t[];
   const int node = lnodes[blockIdx.x];
   const int p = S.npiv[node], m = S.m[node];
-  if (p == 0) return;
   const double* L = S.rpool + S.foff[node];
   const int* vars = S.ipool + S.voff[node];
   const int ld = m, tid = threadIdx.x;
+  const int ns = P.ns[node], D = S.D[node];
+  const int kb = P.kptr[node], ke = P.kptr[node + 1];
+  const int* fpos = S.fpos + S.voff[node];
   for (int r = 0; r < nrhs; r++) {
     double* xr = x + (size_t)r * ldx;
-    for (int j = tid; j < p; j += NT) {
-      int v = vars[j];
-      t[j] = xr[v >= 0 ? v : -v - 1];
+    double* ur = ub + (size_t)r * ubld;
+    for (int j = tid; j < m; j += NT) {
+      const int v = vars[j];
+      t[j] = j < p ? xr[v >= 0 ? v : -v - 1] : 0.0;
     }
     __syncthreads();
+    int doff = ns;
+    for (int q = kb; q < ke; q++) {
+      const int c = P.kids[q];
+      const int pc = S.npiv[c], dc = S.ndel[c], wc = S.m[c] - pc;
+      const double* uc = ur + S.voff[c] + pc;
+      for (int ii = tid; ii < wc; ii += NT) {
+        int pos = child_row_pos(P, c, dc, doff, ns, D, ii);
+        if (pos < ns + D) pos = fpos[pos];
+        t[pos] += uc[ii];
+      }
+      doff += dc;
+      __syncthreads();
+    }
     for (int j = 0; j < p;) {
       if (vars[j] >= 0) {
         const double tj = t[j];
@@ -973,11 +1114,13 @@ __global__ __launch_bounds__(NT) void solve_fwd_kernel(DevState S,
       }
       __syncthreads();
     }
+    double* uo = ur + S.voff[node];
     for (int i = p + tid; i < m; i += NT) {
-      double s = 0.0;
-      for (int j = 0; j < p; j++) s += L[i + (size_t)j * ld] * t[j];
-      atomicAdd(&xr[vars[i]], -s);
+      double s = t[i];
+      for (int j = 0; j < p; j++) s -= L[i + (size_t)j * ld] * t[j];
+      uo[i] = s;
     }
+    __syncthreads();  // t[0:p] is read above and rescaled below
     if (tid == 0) {
       for (int i = 0; i < p;) {
         if (vars[i] >= 0) {
@@ -1090,7 +1233,9 @@ LdlNumeric::~LdlNumeric() {
   for (void* p : allocs_) (void)hipFree(p);
   if (S_.rpool) (void)hipFree(S_.rpool);
   if (S_.ipool) (void)hipFree(S_.ipool);
+  if (S_.fpos) (void)hipFree(S_.fpos);
   if (W_) (void)hipFree(W_);
+  if (ub_) (void)hipFree(ub_);
 }
 
 void LdlNumeric::tk_begin(hipStream_t s) {
@@ -1242,6 +1387,8 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   big_nodes_.clear();
   big_npanels_.assign(nlev, 0);
   big_tiles_.assign(nlev, 0);
+  big_maxkids_.assign(nlev, 0);
+  h_kptr_ = plan.children_ptr;
   std::vector<int> big_root, big_ns;
   std::vector<int64_t> big_woff;
   int64_t wmax = 0;
@@ -1258,6 +1405,8 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
         big_woff.push_back(wtot);
         wtot += (int64_t)mcap * NB;
         big_npanels_[l] = std::max(big_npanels_[l], (ns[k] + DCAP_BIG) / (NB - 1) + 2);
+        big_maxkids_[l] = std::max(big_maxkids_[l],
+                                   plan.children_ptr[k + 1] - plan.children_ptr[k]);
         big_tiles_[l] = std::max(big_tiles_[l], (mcap + TILE - 1) / TILE);
       }
       mx = std::max(mx, std::min(ns[k] + cs[k] + 32, LDS_MAX_M));
@@ -1364,9 +1513,12 @@ int LdlNumeric::ensure_pools(unsigned long long rcap, unsigned long long icap,
   }
   if (icap > S_.icap) {
     if (S_.ipool) HIPCHK(hipFree(S_.ipool));
+    if (S_.fpos) HIPCHK(hipFree(S_.fpos));
     S_.ipool = nullptr;
+    S_.fpos = nullptr;
     S_.icap = 0;
     hipError_t e = hipMalloc(&S_.ipool, icap * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&S_.fpos, icap * sizeof(int));
     if (e != hipSuccess) {
       err = "front index pool allocation failed";
       return -5;
@@ -1416,7 +1568,9 @@ int LdlNumeric::run_factor(const double* vals, const double* shift,
       int flag = 0;
       HIPCHK(hipMemcpy(&flag, S_.flags, sizeof(int), hipMemcpyDeviceToHost));
       if (flag != 0) return flag;
-      std::vector<int> hm(K_), hD(K_);
+      std::vector<int>& hm = h_m_;
+      std::vector<int> hD(K_);
+      hm.resize(K_);
       HIPCHK(hipMemcpy(hm.data(), S_.m, K_ * sizeof(int), hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(hD.data(), S_.D, K_ * sizeof(int), hipMemcpyDeviceToHost));
       std::vector<int> ln, lr, lns;
@@ -1453,6 +1607,29 @@ int LdlNumeric::run_factor(const double* vals, const double* shift,
       bns = d_sync_ns_;
       woff = d_big_woff_sync_;
     }
+    {
+      int maxkids = 0;
+      if (sync_mode) {
+        for (int t = level_ptr_[l]; t < level_ptr_[l + 1]; t++) {
+          const int node = level_nodes_[t];
+          if (h_m_[node] > LDS_MAX_M)
+            maxkids = std::max(maxkids, h_kptr_[node + 1] - h_kptr_[node]);
+        }
+      } else {
+        maxkids = big_maxkids_[l];
+      }
+      const double* scp = scaling_ ? scale_ : nullptr;
+      tk_begin(stream);
+      hipLaunchKernelGGL(big_zero_kernel, dim3(128, nlaunch), dim3(NT), 0, stream, P_, S_,
+                         bnodes, shift, scp);
+      hipLaunchKernelGGL(big_orig_kernel, dim3(16, nlaunch), dim3(NT), 0, stream, P_, S_,
+                         bnodes, vals, scp);
+      for (int q = 0; q < maxkids; q++)
+        hipLaunchKernelGGL(big_child_kernel, dim3(128, nlaunch), dim3(NT), 0, stream, P_,
+                           S_, bnodes, q);
+      tk_end(stream, TK_ASSEMBLE);
+      HIPCHK(hipGetLastError());
+    }
     for (int q = 0; q < npanels; q++) {
       tk_begin(stream);
       hipLaunchKernelGGL(panel_kernel, dim3(nlaunch), dim3(NT), 0, stream, S_,
@@ -1468,6 +1645,8 @@ int LdlNumeric::run_factor(const double* vals, const double* shift,
                        stream, S_, bnodes, bns, nlaunch);
     HIPCHK(hipGetLastError());
   }
+  hipLaunchKernelGGL(fpos_kernel, dim3(std::min(K_, 4096)), dim3(NT), 0, stream, P_, S_, K_);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(stream));
   if (timing_) tk_collect();
   int flags[8];
@@ -1537,6 +1716,17 @@ int LdlNumeric::solve(int nrhs, double* x, int ldx, hipStream_t stream,
     return -1;
   }
   const int nlev = num_levels();
+  const size_t ubneed = (size_t)S_.icap * std::max(nrhs, 1);
+  if (ubneed > ub_cap_) {
+    if (ub_) HIPCHK(hipFree(ub_));
+    ub_ = nullptr;
+    ub_cap_ = 0;
+    if (hipMalloc(&ub_, ubneed * sizeof(double)) != hipSuccess) {
+      err = "solve workspace allocation failed";
+      return -5;
+    }
+    ub_cap_ = ubneed;
+  }
   const int sb = std::min(4096, std::max(1, (n_ * std::max(nrhs, 1) + NT - 1) / NT));
   if (scaling_)
     hipLaunchKernelGGL(scale_vec_kernel, dim3(sb), dim3(NT), 0, stream, n_, nrhs, ldx,
@@ -1544,8 +1734,8 @@ int LdlNumeric::solve(int nrhs, double* x, int ldx, hipStream_t stream,
   for (int l = 0; l < nlev; l++) {
     const int cnt = level_ptr_[l + 1] - level_ptr_[l];
     tk_begin(stream);
-    hipLaunchKernelGGL(solve_fwd_kernel, dim3(cnt), dim3(NT), lds, stream, S_,
-                       d_level_nodes_ + level_ptr_[l], x, ldx, nrhs);
+    hipLaunchKernelGGL(solve_fwd_kernel, dim3(cnt), dim3(NT), lds, stream, P_, S_,
+                       d_level_nodes_ + level_ptr_[l], x, ldx, nrhs, ub_, (size_t)S_.icap);
     tk_end(stream, TK_SOLVE_FWD);
   }
   for (int l = nlev - 1; l >= 0; l--) {

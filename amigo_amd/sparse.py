@@ -251,11 +251,11 @@ class SparseLDL:
 
     def timing(self):
         """{class: (ms, launches)} and the flops split (small, big)."""
-        ms = (ctypes.c_double * 5)()
-        cnt = (ctypes.c_longlong * 5)()
+        ms = (ctypes.c_double * 6)()
+        cnt = (ctypes.c_longlong * 6)()
         fs, fb = ctypes.c_double(), ctypes.c_double()
         _lib.check(_lib.lib().amigo_ldl_get_timing(self._h, ms, cnt, ctypes.byref(fs), ctypes.byref(fb)))
-        names = ["level", "panel", "schur_update", "solve_fwd", "solve_bwd"]
+        names = ["level", "panel", "schur_update", "solve_fwd", "solve_bwd", "assemble_big"]
         return {k: (ms[i], cnt[i]) for i, k in enumerate(names)}, (fs.value, fb.value)
 
     def work(self):

@@ -101,7 +101,11 @@ def main():
     last = outs[-1]["iterations"][-1]
     iters = outs[-1]["num_iterations"]
 
-    # Dominant kernel class and its algorithmic work.
+    # Dominant kernel class and its algorithmic work (the blocked-front
+    # assembly is counted with the level kernels, which assemble the rest).
+    times = dict(times)
+    asm_ms, asm_n = times.pop("assemble_big")
+    times["level"] = (times["level"][0] + asm_ms, times["level"][1])
     cls = max(times, key=lambda k: times[k][0])
     ms, launches = times[cls]
     if cls == "level":
@@ -156,7 +160,7 @@ def main():
         "final_nlp_error": last["nlp_error"],
         "factorizations_per_solve": nfact,
         "linear_solves_per_solve": nsolve,
-        "kernel_ms": {k: v[0] for k, v in times.items()},
+        "kernel_ms": dict({k: v[0] for k, v in times.items()}, assemble_big=asm_ms),
         "roofline": roof,
     }
     if not args.no_cpu_baseline:

@@ -131,10 +131,11 @@ int amigo_ldl_set_scaling(amigo_ldl* h, int mode);
 /* HIP-event timing of every factor/solve launch (on = 1; 2 also resets
  * the totals), and the accumulated milliseconds / launch counts per kernel
  * class: [level (assemble + workgroup factor), panel, Schur update,
- * forward solve, backward solve], plus the algorithmic flops of the fronts
- * each factorization path owns.  No reference counterpart. */
+ * forward solve, backward solve, blocked-front assembly], plus the
+ * algorithmic flops of the fronts each factorization path owns.  No
+ * reference counterpart. */
 int amigo_ldl_set_timing(amigo_ldl* h, int on);
-int amigo_ldl_get_timing(const amigo_ldl* h, double* ms5, long long* count5,
+int amigo_ldl_get_timing(const amigo_ldl* h, double* ms6, long long* count6,
                          double* flops_small, double* flops_big);
 
 /* Total delayed pivots and largest front of the last factorization (the
