@@ -33,6 +33,8 @@ constexpr int BIG_M = 128;       // static fronts above this use the blocked pat
 constexpr int TILE = 64;         // Schur-update tile
 constexpr int DCAP_BIG = 64;     // delayed pivots a blocked front is sized for
 constexpr int PS = 8;            // ints of blocked-path state per front
+constexpr int NTP = 1024;        // threads of the panel workgroup (16 wave64)
+constexpr int NBW = NB + 2;      // W columns: NB pivots, a 2x2 overhang, imax
 
 struct VI {
   double v;
@@ -70,27 +72,63 @@ __device__ __forceinline__ void swapd(double& a, double& b) {
 
 // Symmetric interchange of rows/columns a < b of the lower triangle
 // (sparse_ldl.h:1252-1274).  Ends with a barrier.
-__device__ void symm_swap(double* F, int ld, int m, int* vars, int a, int b) {
+template <int NTH>
+__device__ void symm_swap_t(double* F, int ld, int m, int* vars, int a, int b) {
   const int t0 = threadIdx.x;
-  for (int t = t0; t < a; t += NT) swapd(F[a + (size_t)t * ld], F[b + (size_t)t * ld]);
+  for (int t = t0; t < a; t += NTH) swapd(F[a + (size_t)t * ld], F[b + (size_t)t * ld]);
   if (t0 == 0) {
     swapd(F[a + (size_t)a * ld], F[b + (size_t)b * ld]);
     int v = vars[a];
     vars[a] = vars[b];
     vars[b] = v;
   }
-  for (int t = a + 1 + t0; t < b; t += NT)
+  for (int t = a + 1 + t0; t < b; t += NTH)
     swapd(F[t + (size_t)a * ld], F[b + (size_t)t * ld]);
-  for (int t = b + 1 + t0; t < m; t += NT)
+  for (int t = b + 1 + t0; t < m; t += NTH)
     swapd(F[t + (size_t)a * ld], F[t + (size_t)b * ld]);
   __syncthreads();
 }
+__device__ void symm_swap(double* F, int ld, int m, int* vars, int a, int b) {
+  symm_swap_t<NT>(F, ld, m, vars, a, b);
+}
 
 // Swap rows a and b of W for columns [0, ncol).
-__device__ void swap_w_rows(double* W, int ldw, int a, int b, int ncol) {
-  for (int q = threadIdx.x; q < ncol; q += NT)
+template <int NTH>
+__device__ void swap_w_rows_t(double* W, int ldw, int a, int b, int ncol) {
+  for (int q = threadIdx.x; q < ncol; q += NTH)
     swapd(W[a + (size_t)q * ldw], W[b + (size_t)q * ldw]);
   __syncthreads();
+}
+
+// Two block-wide argmax reductions in one pass (NTH threads; red holds
+// 2 * NTH / 64 entries).  Results go to all threads.
+template <int NTH>
+__device__ void block_argmax2(VI& x, VI& y, VI* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    VI o, p;
+    o.v = __shfl_xor(x.v, off, 64);
+    o.i = __shfl_xor(x.i, off, 64);
+    p.v = __shfl_xor(y.v, off, 64);
+    p.i = __shfl_xor(y.i, off, 64);
+    x = vi_best(x, o);
+    y = vi_best(y, p);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    red[wid] = x;
+    red[NTH / 64 + wid] = y;
+  }
+  __syncthreads();
+  VI r = red[0], q = red[NTH / 64];
+#pragma unroll
+  for (int w = 1; w < NTH / 64; w++) {
+    r = vi_best(r, red[w]);
+    q = vi_best(q, red[NTH / 64 + w]);
+  }
+  x = r;
+  y = q;
 }
 
 // Pivot decision shared by the unblocked and the panel paths.  colk(i) and
@@ -722,21 +760,47 @@ __global__ __launch_bounds__(NT) void big_child_kernel(DevPlan P, DevState S,
   (void)m;
 }
 
+// s - sum_q Fi[q * ld] * wr[q] for q < kw, four independent partial sums so
+// that the strided loads of the panel columns overlap.
+__device__ __forceinline__ double panel_dot(double s, const double* __restrict__ Fi,
+                                            size_t ld, const double* wr, int kw) {
+  double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int q = 0;
+#pragma unroll 2
+  for (; q + 4 <= kw; q += 4) {
+    const double f0 = Fi[(size_t)q * ld], f1 = Fi[(size_t)(q + 1) * ld];
+    const double f2 = Fi[(size_t)(q + 2) * ld], f3 = Fi[(size_t)(q + 3) * ld];
+    s -= f0 * wr[q];
+    s1 -= f1 * wr[q + 1];
+    s2 -= f2 * wr[q + 2];
+    s3 -= f3 * wr[q + 3];
+  }
+  for (; q < kw; q++) s -= Fi[(size_t)q * ld] * wr[q];
+  return (s + s1) + (s2 + s3);
+}
+
 // ---------------------------------------------------------------------------
-// Blocked path for large fronts.  One panel: up to NB-1 pivots selected with
-// the reference's left-looking panel loop (sparse_ldl.h:1309-1505), W holds
-// the unscaled updated columns (m x NB, ld = m).
-__global__ __launch_bounds__(NT) void panel_kernel(DevState S,
-                                                   const int* __restrict__ bnodes,
-                                                   const int* __restrict__ broot,
-                                                   double* __restrict__ Wbase,
-                                                   const int64_t* __restrict__ woff,
-                                                   double ustab,
-                                                   double pivot_tol, int chol) {
+// Blocked path for large fronts.  One panel: up to NB pivots selected with
+// the reference's left-looking panel loop (sparse_ldl.h:1309-1505); W holds
+// the unscaled updated columns (m x NBW, ld = m).  The reference's panel
+// holds nblock - 1 = 63 pivots; the choice of panel width does not change
+// any pivot decision.  One 1024-thread workgroup per front: each column
+// update W[k:, kw] = F[k:, k] - F[k:, kstart:k] W[k, :kw]^T reads row k of W
+// from LDS, and the candidate / contribution maxima come out of the same
+// pass with one fused reduction.
+__global__ __launch_bounds__(NTP) void panel_kernel(DevState S,
+                                                    const int* __restrict__ bnodes,
+                                                    const int* __restrict__ broot,
+                                                    double* __restrict__ Wbase,
+                                                    const int64_t* __restrict__ woff,
+                                                    double ustab,
+                                                    double pivot_tol, int chol) {
   const int node = bnodes[blockIdx.x];
   const int root = broot[blockIdx.x];
   double* W = Wbase + woff[blockIdx.x];
-  __shared__ VI red[NT / 64];
+  __shared__ VI red[2 * NTP / 64];
+  __shared__ double wrow[NBW];
+  __shared__ double sval[2];
   __shared__ int cnt[4];
   __shared__ int bad;
   const int tid = threadIdx.x;
@@ -744,6 +808,7 @@ __global__ __launch_bounds__(NT) void panel_kernel(DevState S,
   __syncthreads();
   if (bad != 0) return;
   int* ps = S.pstate + PS * node;
+  if (ps[4] != 1) return;  // factored in a workgroup of the level kernel
   const int m = S.m[node];
   double* F = S.rpool + S.foff[node];
   int* vars = S.ipool + S.voff[node];
@@ -761,31 +826,34 @@ __global__ __launch_bounds__(NT) void panel_kernel(DevState S,
   const int kstart = k;
   int kw = 0;
   const bool bk = root != 0;
-  while (kw < NB - 1 && k < nc) {
-    // W[k:, kw] = F[k:, k] - F[k:, kstart:k] W[k, :kw]^T
-    for (int i = k + tid; i < m; i += NT) {
-      double s = F[i + (size_t)k * ld];
-      for (int q = 0; q < kw; q++)
-        s -= F[i + (size_t)(kstart + q) * ld] * W[k + (size_t)q * ldw];
-      W[i + (size_t)kw * ldw] = s;
-    }
-    __syncthreads();
+  while (kw < NB && k < nc) {
     double* wk = W + (size_t)kw * ldw;
     double* wi = W + (size_t)(kw + 1) * ldw;
-    int step = 1, kpiv = k;
-    bool zero = false;
-    if (!chol) {
-      VI a = {-1.0, INT_MAX}, b = {-1.0, INT_MAX};
-      for (int i = k + 1 + tid; i < m; i += NT) {
-        VI c = {fabs(wk[i]), i};
+    // Row k of W (the multipliers of this column's update) into LDS.
+    if (tid < kw) wrow[tid] = W[k + (size_t)tid * ldw];
+    __syncthreads();
+    VI a = {-1.0, INT_MAX}, b = {-1.0, INT_MAX};
+    for (int i = k + tid; i < m; i += NTP) {
+      const double s = panel_dot(F[i + (size_t)k * ld], F + i + (size_t)kstart * ld,
+                                 (size_t)ld, wrow, kw);
+      wk[i] = s;
+      if (i == k) sval[0] = s;
+      if (i > k) {
+        VI c = {fabs(s), i};
         if (i < nc)
           a = vi_best(a, c);
         else
           b = vi_best(b, c);
       }
-      const VI ca = block_argmax(a, red);
-      const VI cb = block_argmax(b, red);
-      const double akk = fabs(wk[k]);
+    }
+    int step = 1, kpiv = k;
+    bool zero = false;
+    if (chol) {
+      __syncthreads();
+    } else {
+      block_argmax2<NTP>(a, b, red);
+      const VI ca = a, cb = b;
+      const double akk = fabs(sval[0]);
       const bool has_cand = k + 1 < nc && ca.i > k && ca.i < nc;
       const int imax = has_cand ? ca.i : 0;
       const double abswik = has_cand ? ca.v : 0.0;
@@ -797,21 +865,23 @@ __global__ __launch_bounds__(NT) void panel_kernel(DevState S,
         kpiv = k;
         zero = bk && fmax(akk, colmax) == 0.0;
       } else if (has_cand) {
-        // Column imax, updated: W[k:, kw+1].
-        for (int i = k + tid; i < m; i += NT) {
-          double s = i < imax ? F[imax + (size_t)i * ld] : F[i + (size_t)imax * ld];
-          for (int q = 0; q < kw; q++)
-            s -= F[i + (size_t)(kstart + q) * ld] * W[imax + (size_t)q * ldw];
-          wi[i] = s;
-        }
+        // Column imax, updated: W[k:, kw+1]; its largest off-diagonal entry
+        // comes out of the same pass.
+        if (tid < kw) wrow[tid] = W[imax + (size_t)tid * ldw];
         __syncthreads();
-        VI r = {-1.0, INT_MAX};
-        for (int i = k + tid; i < m; i += NT) {
-          if (i == imax) continue;
-          r = vi_best(r, VI{fabs(wi[i]), i});
+        VI r = {-1.0, INT_MAX}, dummy = {-1.0, INT_MAX};
+        for (int i = k + tid; i < m; i += NTP) {
+          const double s0 = i < imax ? F[imax + (size_t)i * ld] : F[i + (size_t)imax * ld];
+          const double s = panel_dot(s0, F + i + (size_t)kstart * ld, (size_t)ld, wrow, kw);
+          wi[i] = s;
+          if (i == imax)
+            sval[1] = s;
+          else
+            r = vi_best(r, VI{fabs(s), i});
         }
-        const double rowmax = block_argmax(r, red).v;
-        const double aii = fabs(wi[imax]);
+        block_argmax2<NTP>(r, dummy, red);
+        const double rowmax = r.v;
+        const double aii = fabs(sval[1]);
         if (bk) {
           if (akk >= BK_ALPHA * colmax * (colmax / rowmax)) {
             step = 1;
@@ -827,7 +897,7 @@ __global__ __launch_bounds__(NT) void panel_kernel(DevState S,
           step = 1;
           kpiv = imax;
         } else {
-          const double wkk = wk[k], wik = wk[imax], wii = wi[imax];
+          const double wkk = sval[0], wik = wk[imax], wii = sval[1];
           const double det = fabs(wkk * wii - wik * wik);
           step = 0;
           if (det >= pivot_tol) {
@@ -839,7 +909,7 @@ __global__ __launch_bounds__(NT) void panel_kernel(DevState S,
           }
         }
         if (step == 1 && kpiv == imax) {
-          for (int i = k + tid; i < m; i += NT) wk[i] = wi[i];
+          for (int i = k + tid; i < m; i += NTP) wk[i] = wi[i];
           __syncthreads();
         }
       } else {
@@ -849,24 +919,24 @@ __global__ __launch_bounds__(NT) void panel_kernel(DevState S,
 
     if (step == 0) {
       if (k < nc - 1) {
-        symm_swap(F, ld, m, vars, k, nc - 1);
-        swap_w_rows(W, ldw, k, nc - 1, kw);
+        symm_swap_t<NTP>(F, ld, m, vars, k, nc - 1);
+        swap_w_rows_t<NTP>(W, ldw, k, nc - 1, kw);
       }
       nc -= 1;
       continue;
     }
     const int ksrc = k + step - 1;
     if (kpiv != ksrc) {
-      symm_swap(F, ld, m, vars, ksrc, kpiv);
-      swap_w_rows(W, ldw, ksrc, kpiv, kw + step);
+      symm_swap_t<NTP>(F, ld, m, vars, ksrc, kpiv);
+      swap_w_rows_t<NTP>(W, ldw, ksrc, kpiv, kw + step);
     }
     if (step == 1) {
       const double d = wk[k];
       if (zero) {
-        for (int i = k + tid; i < m; i += NT) F[i + (size_t)k * ld] = 0.0;
+        for (int i = k + tid; i < m; i += NTP) F[i + (size_t)k * ld] = 0.0;
       } else {
         const double dinv = 1.0 / d;
-        for (int i = k + tid; i < m; i += NT)
+        for (int i = k + tid; i < m; i += NTP)
           F[i + (size_t)k * ld] = (i == k) ? wk[i] : wk[i] * dinv;
       }
       if (tid == 0) count_inertia(1, d, 0.0, 0.0, zero, cnt);
@@ -874,7 +944,7 @@ __global__ __launch_bounds__(NT) void panel_kernel(DevState S,
       const double d11 = wk[k], d21 = wk[k + 1], d22 = wi[k + 1];
       const double detinv = 1.0 / (d11 * d22 - d21 * d21);
       const double i00 = d22 * detinv, i01 = -d21 * detinv, i11 = d11 * detinv;
-      for (int i = k + 2 + tid; i < m; i += NT) {
+      for (int i = k + 2 + tid; i < m; i += NTP) {
         const double w1 = wk[i], w2 = wi[i];
         F[i + (size_t)k * ld] = w1 * i00 + w2 * i01;
         F[i + (size_t)(k + 1) * ld] = w1 * i01 + w2 * i11;
@@ -912,8 +982,8 @@ __global__ __launch_bounds__(NT) void trailing_kernel(DevState S,
                                                       const int64_t* __restrict__ woff) {
   const int node = bnodes[blockIdx.z];
   const double* W = Wbase + woff[blockIdx.z];
-  __shared__ double As[TILE][NB + 1];
-  __shared__ double Bs[TILE][NB + 1];
+  __shared__ double As[TILE][NBW + 1];
+  __shared__ double Bs[TILE][NBW + 1];
   __shared__ int bad;
   if (threadIdx.x == 0) bad = atomicAdd(&S.flags[0], 0);
   __syncthreads();
@@ -1403,8 +1473,8 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
         big_ns.push_back(ns[k]);
         const int mcap = ns[k] + cs[k] + DCAP_BIG;
         big_woff.push_back(wtot);
-        wtot += (int64_t)mcap * NB;
-        big_npanels_[l] = std::max(big_npanels_[l], (ns[k] + DCAP_BIG) / (NB - 1) + 2);
+        wtot += (int64_t)mcap * NBW;
+        big_npanels_[l] = std::max(big_npanels_[l], (ns[k] + DCAP_BIG + NB - 1) / NB);
         big_maxkids_[l] = std::max(big_maxkids_[l],
                                    plan.children_ptr[k + 1] - plan.children_ptr[k]);
         big_tiles_[l] = std::max(big_tiles_[l], (mcap + TILE - 1) / TILE);
@@ -1585,8 +1655,8 @@ int LdlNumeric::run_factor(const double* vals, const double* shift,
         lr.push_back(h_cs_[node] == 0 ? 1 : 0);
         lns.push_back(h_ns_[node]);
         wo.push_back(tot);
-        tot += (int64_t)hm[node] * NB;
-        npanels = std::max(npanels, (h_ns_[node] + hD[node]) / (NB - 1) + 2);
+        tot += (int64_t)hm[node] * NBW;
+        npanels = std::max(npanels, (h_ns_[node] + hD[node] + NB - 1) / NB);
         tiles = std::max(tiles, (hm[node] + TILE - 1) / TILE);
       }
       nlaunch = int(ln.size());
@@ -1632,7 +1702,7 @@ int LdlNumeric::run_factor(const double* vals, const double* shift,
     }
     for (int q = 0; q < npanels; q++) {
       tk_begin(stream);
-      hipLaunchKernelGGL(panel_kernel, dim3(nlaunch), dim3(NT), 0, stream, S_,
+      hipLaunchKernelGGL(panel_kernel, dim3(nlaunch), dim3(NTP), 0, stream, S_,
                          bnodes, broot, W_, woff, prm.ustab, prm.pivot_tol,
                          cholesky ? 1 : 0);
       tk_end(stream, TK_PANEL);

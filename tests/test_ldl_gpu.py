@@ -139,3 +139,74 @@ def test_shifted_factor():
     assert ldl.get_inertia() == (int(np.sum(ev >= 0)), int(np.sum(ev < 0)))
     r, _ = _residual(B, ldl)
     assert r < 1e-10
+
+
+def _dense_block_tail(nb=200, tail=60, seed=1):
+    from scipy.sparse import csr_matrix
+
+    rng = np.random.default_rng(seed)
+    B = rng.normal(size=(nb, nb))
+    S = B @ B.T + nb * np.eye(nb)
+    n = nb + tail
+    A = np.zeros((n, n))
+    A[:nb, :nb] = S
+    for i in range(nb, n):
+        A[i, i] = 4.0
+        A[i, i - 1] = A[i - 1, i] = 1.0
+    return csr_matrix(A)
+
+
+@pytest.mark.parametrize("case", ["root", "nonroot", "nonroot_shift", "indefinite"])
+def test_blocked_path_fronts(case):
+    """Fronts above the workgroup limit (blocked panel path + grid-wide
+    assembly): a dense root front, a dense non-root front, the fused
+    diagonal shift, and an indefinite dense front with 2x2 pivots."""
+    from amigo_amd.sparse import CSRMat, SparseLDL
+    from scipy.sparse import csr_matrix
+
+    if case == "root":
+        A = _dense_block_tail(300, 0)
+    elif case == "indefinite":
+        rng = np.random.default_rng(3)
+        B = rng.normal(size=(260, 260))
+        A = csr_matrix(B + B.T)
+    else:
+        A = _dense_block_tail(200, 60)
+    A.sort_indices()
+    ldl = SparseLDL(CSRMat.from_scipy(A), order="NATURAL")
+    Ad = A.toarray()
+    if case == "nonroot_shift":
+        sh = np.linspace(0.1, 2.0, A.shape[0])
+        ldl.factor(shift=torch.tensor(sh, device="cuda"))
+        Ad = Ad + np.diag(sh)
+    else:
+        ldl.factor()
+    assert ldl.stats()[1] > 128
+    ev = np.linalg.eigvalsh(Ad)
+    assert ldl.get_inertia() == (int(np.sum(ev >= 0)), int(np.sum(ev < 0)))
+    b = np.random.default_rng(0).normal(size=A.shape[0])
+    x = torch.tensor(b, device="cuda")
+    ldl.solve(x)
+    r = np.linalg.norm(Ad @ x.cpu().numpy() - b) / np.linalg.norm(b)
+    assert r < 1e-10
+
+
+def test_delays_with_wide_fronts_match_oracle():
+    """Elliptic-control KKT with many delayed pivots: per-node pivot and
+    delay counts equal the oracle restatement's on the same plan."""
+    from amigo_amd.sparse import CSRMat, SparseLDL
+    from oracle.ldl import OracleLDL
+
+    A = M.grid_kkt(16, seed=2)[0]
+    ldl = SparseLDL(CSRMat.from_scipy(A), order="AMD", ustab=0.1)
+    ldl.factor()
+    O = OracleLDL(A.shape[0], A.indptr, A.indices, ldl.get_symbolic(), ustab=0.1)
+    O.factor(A.data)
+    npiv, ndel = ldl.pivot_counts()
+    assert ldl.get_inertia() == O.get_inertia()
+    assert np.array_equal(npiv, O.npiv) and np.array_equal(ndel, O.ndel)
+    assert ndel.sum() > 50
+    b = np.random.default_rng(1).normal(size=A.shape[0])
+    x = torch.tensor(b, device="cuda")
+    ldl.solve(x)
+    assert np.allclose(x.cpu().numpy(), O.solve(b), rtol=1e-9, atol=1e-9)
