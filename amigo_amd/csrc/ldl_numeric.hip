@@ -32,9 +32,11 @@ constexpr int LDS_MAX_M = 128;   // largest front held in LDS (128 KiB)
 constexpr int BIG_M = 128;       // static fronts above this use the blocked path
 constexpr int TILE = 64;         // Schur-update tile
 constexpr int DCAP_BIG = 64;     // delayed pivots a blocked front is sized for
+constexpr int DPAN_BIG = 16;     // delayed pivots the static panel count covers
 constexpr int PS = 8;            // ints of blocked-path state per front
 constexpr int NTP = 1024;        // threads of the panel workgroup (16 wave64)
 constexpr int NBW = NB + 2;      // W columns: NB pivots, a 2x2 overhang, imax
+constexpr int NBR = 16;          // pivots per panel when the panel rows fit in registers
 
 struct VI {
   double v;
@@ -325,6 +327,67 @@ __device__ int factor_unblocked(double* F, int ld, int m, int nfs, int* vars,
 // Column statistics of column k in one pass: the largest candidate entry
 // (rows k+1..nc-1, with its row) and the largest contribution entry (rows
 // nc..m-1).  The column is cached in Wv on the way.  Two barriers.
+// Order-preserving key of a candidate |value| for max reductions: 0 for
+// "no candidate" (and NaN, which never wins, as in vi_best), otherwise the
+// bit pattern of the non-negative double plus one.
+__device__ __forceinline__ unsigned long long cand_key(VI c) {
+  if (!(c.v >= 0.0)) return 0ull;
+  return (unsigned long long)__double_as_longlong(c.v) + 1ull;
+}
+
+// Wave-wide maximum of a 64-bit key with DPP row shifts and row broadcasts
+// (the result is in lane 63, read back to every lane).
+#define AMIGO_DPP_MAX_STEP(x, ctrl, rm, bm)                                                  \
+  {                                                                                          \
+    const unsigned lo_ = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(x), ctrl, \
+                                                               rm, bm, false);               \
+    const unsigned hi_ = (unsigned)__builtin_amdgcn_update_dpp(                              \
+        0, (int)(unsigned)((x) >> 32), ctrl, rm, bm, false);                                 \
+    const unsigned long long y_ = ((unsigned long long)hi_ << 32) | lo_;                     \
+    x = y_ > x ? y_ : x;                                                                     \
+  }
+__device__ __forceinline__ unsigned long long wave_max_key(unsigned long long x) {
+  AMIGO_DPP_MAX_STEP(x, 0x111, 0xf, 0xf)  // row_shr:1
+  AMIGO_DPP_MAX_STEP(x, 0x112, 0xf, 0xf)  // row_shr:2
+  AMIGO_DPP_MAX_STEP(x, 0x113, 0xf, 0xf)  // row_shr:3
+  AMIGO_DPP_MAX_STEP(x, 0x114, 0xf, 0xe)  // row_shr:4
+  AMIGO_DPP_MAX_STEP(x, 0x118, 0xf, 0xc)  // row_shr:8
+  AMIGO_DPP_MAX_STEP(x, 0x142, 0xa, 0xf)  // row_bcast:15
+  AMIGO_DPP_MAX_STEP(x, 0x143, 0xc, 0xf)  // row_bcast:31
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), 63);
+  return ((unsigned long long)hi << 32) | lo;
+}
+#undef AMIGO_DPP_MAX_STEP
+
+// Two block-wide argmax reductions (largest value, smallest index on ties;
+// lanes hold at most one candidate each, and a lane's index grows with the
+// lane id inside a wave) -> all threads.  red holds 2 * NTH / 64 entries.
+// The caller guarantees a barrier between consecutive calls.
+template <int NTH>
+__device__ void block_argmax2_dpp(VI& x, VI& y, VI* red) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const unsigned long long kx = cand_key(x), ky = cand_key(y);
+  const unsigned long long mx = wave_max_key(kx), my = wave_max_key(ky);
+  const unsigned long long bx = __ballot(kx == mx && mx != 0ull);
+  const unsigned long long by = __ballot(ky == my && my != 0ull);
+  const int ix = bx ? __builtin_amdgcn_readlane(x.i, __ffsll((long long)bx) - 1) : INT_MAX;
+  const int iy = by ? __builtin_amdgcn_readlane(y.i, __ffsll((long long)by) - 1) : INT_MAX;
+  if (lane == 0) {
+    red[wid] = VI{mx ? __longlong_as_double((long long)(mx - 1ull)) : -1.0, ix};
+    red[NTH / 64 + wid] = VI{my ? __longlong_as_double((long long)(my - 1ull)) : -1.0, iy};
+  }
+  __syncthreads();
+  VI r = red[0], q = red[NTH / 64];
+#pragma unroll
+  for (int w = 1; w < NTH / 64; w++) {
+    r = vi_best(r, red[w]);
+    q = vi_best(q, red[NTH / 64 + w]);
+  }
+  x = r;
+  y = q;
+}
+
 struct ColStats {
   double cv;
   int ci;
@@ -332,47 +395,21 @@ struct ColStats {
 };
 
 __device__ ColStats block_colstats(const double* F, int ld, int m, int k, int nc,
-                                   double* Wv, ColStats* red) {
-  ColStats x = {-1.0, INT_MAX, -1.0};
-  for (int i = k + 1 + threadIdx.x; i < m; i += NT) {
+                                   double* Wv, VI* red) {
+  // One row per thread (fronts of at most LDS_MAX_M < NT rows), so the
+  // wave reductions can use DPP and a ballot (row order = lane order).
+  VI c = {-1.0, INT_MAX}, b = {-1.0, INT_MAX};
+  const int i = k + 1 + threadIdx.x;
+  if (i < m) {
     const double w = F[i + (size_t)k * ld];
     Wv[i] = w;
-    const double a = fabs(w);
-    if (i < nc) {
-      if (a > x.cv || (a == x.cv && i < x.ci)) {
-        x.cv = a;
-        x.ci = i;
-      }
-    } else {
-      x.bv = fmax(x.bv, a);
-    }
+    if (i < nc)
+      c = VI{fabs(w), i};
+    else
+      b = VI{fabs(w), i};
   }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const double ov = __shfl_xor(x.cv, off, 64);
-    const int oi = __shfl_xor(x.ci, off, 64);
-    const double ob = __shfl_xor(x.bv, off, 64);
-    if (ov > x.cv || (ov == x.cv && oi < x.ci)) {
-      x.cv = ov;
-      x.ci = oi;
-    }
-    x.bv = fmax(x.bv, ob);
-  }
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) red[wid] = x;
-  __syncthreads();
-  ColStats r = red[0];
-#pragma unroll
-  for (int w = 1; w < NT / 64; w++) {
-    const ColStats o = red[w];
-    if (o.cv > r.cv || (o.cv == r.cv && o.ci < r.ci)) {
-      r.cv = o.cv;
-      r.ci = o.ci;
-    }
-    r.bv = fmax(r.bv, o.bv);
-  }
-  return r;
+  block_argmax2_dpp<NT>(c, b, red);
+  return ColStats{c.v, c.i, b.v};
 }
 
 // LDS-resident front: same pivot rules as factor_unblocked, organized for
@@ -381,7 +418,7 @@ __device__ ColStats block_colstats(const double* F, int ld, int m, int k, int nc
 // rank-1 update + scaling pass.
 __device__ int factor_front_lds(double* F, int ld, int m, int nfs, int* vars,
                                 bool bk, bool chol, double ustab, double pivot_tol,
-                                double* Wv, VI* red, ColStats* cred, int* cnt) {
+                                double* Wv, VI* red, VI* cred, int* cnt) {
   const int tid = threadIdx.x;
   int k = 0, nc = nfs;
   while (k < nc) {
@@ -399,13 +436,14 @@ __device__ int factor_front_lds(double* F, int ld, int m, int nfs, int* vars,
       if (ok1) {
         zero = bk && fmax(akk, colmax) == 0.0;
       } else if (has_cand) {
-        VI r = {-1.0, INT_MAX};
-        for (int i = k + tid; i < m; i += NT) {
-          if (i == imax) continue;
+        VI r = {-1.0, INT_MAX}, dummy = {-1.0, INT_MAX};
+        const int i = k + tid;
+        if (i < m && i != imax) {
           const double g = i < imax ? F[imax + (size_t)i * ld] : F[i + (size_t)imax * ld];
-          r = vi_best(r, VI{fabs(g), i});
+          r = VI{fabs(g), i};
         }
-        const double rowmax = block_argmax(r, red).v;
+        block_argmax2_dpp<NT>(r, dummy, red);
+        const double rowmax = r.v;
         const double aii = fabs(F[imax + (size_t)imax * ld]);
         if (bk) {
           if (akk >= BK_ALPHA * colmax * (colmax / rowmax)) {
@@ -454,14 +492,14 @@ __device__ int factor_front_lds(double* F, int ld, int m, int nfs, int* vars,
         __syncthreads();
       }
       if (!zero) {
+        // Rank-1 update of the lower trailing triangle: a wave per column,
+        // lanes down the rows.
         const double dinv = 1.0 / d;
-        const int w = m - k - 1;
-        const int tot = w * w;
-        for (int e = tid; e < tot; e += NT) {
-          const int ii = e % w, jj = e / w;
-          if (ii < jj) continue;
-          const int i = k + 1 + ii, j = k + 1 + jj;
-          F[i + (size_t)j * ld] -= (Wv[i] * dinv) * Wv[j];
+        const int lane = tid & 63, wv = tid >> 6;
+        for (int j = k + 1 + wv; j < m; j += NT / 64) {
+          const double wj = Wv[j];
+          double* Fj = F + (size_t)j * ld;
+          for (int i = j + lane; i < m; i += 64) Fj[i] -= (Wv[i] * dinv) * wj;
         }
         for (int i = k + 1 + tid; i < m; i += NT) F[i + (size_t)k * ld] = Wv[i] * dinv;
       }
@@ -474,22 +512,27 @@ __device__ int factor_front_lds(double* F, int ld, int m, int nfs, int* vars,
       const double c = F[k + 1 + (size_t)(k + 1) * ld];
       const double detinv = 1.0 / (a * c - b * b);
       const double i00 = c * detinv, i01 = -b * detinv, i11 = a * detinv;
-      const int w = m - k - 2;
-      const int tot = w * w;
-      for (int e = tid; e < tot; e += NT) {
-        const int ii = e % w, jj = e / w;
-        if (ii < jj) continue;
-        const int i = k + 2 + ii, j = k + 2 + jj;
+      // L columns of the 2x2 block first (Wv, Wv2), then the rank-2
+      // update of the lower trailing triangle, a wave per column.
+      double* Wv2 = Wv + LDS_MAX_M;
+      for (int i = k + 2 + tid; i < m; i += NT) {
         const double w1 = F[i + (size_t)k * ld], w2 = F[i + (size_t)(k + 1) * ld];
-        const double l1 = w1 * i00 + w2 * i01;
-        const double l2 = w1 * i01 + w2 * i11;
-        F[i + (size_t)j * ld] -= l1 * F[j + (size_t)k * ld] + l2 * F[j + (size_t)(k + 1) * ld];
+        Wv[i] = w1 * i00 + w2 * i01;
+        Wv2[i] = w1 * i01 + w2 * i11;
+      }
+      __syncthreads();
+      {
+        const int lane = tid & 63, wv = tid >> 6;
+        for (int j = k + 2 + wv; j < m; j += NT / 64) {
+          const double a1 = F[j + (size_t)k * ld], a2 = F[j + (size_t)(k + 1) * ld];
+          double* Fj = F + (size_t)j * ld;
+          for (int i = j + lane; i < m; i += 64) Fj[i] -= Wv[i] * a1 + Wv2[i] * a2;
+        }
       }
       __syncthreads();
       for (int i = k + 2 + tid; i < m; i += NT) {
-        const double w1 = F[i + (size_t)k * ld], w2 = F[i + (size_t)(k + 1) * ld];
-        F[i + (size_t)k * ld] = w1 * i00 + w2 * i01;
-        F[i + (size_t)(k + 1) * ld] = w1 * i01 + w2 * i11;
+        F[i + (size_t)k * ld] = Wv[i];
+        F[i + (size_t)(k + 1) * ld] = Wv2[i];
       }
       if (tid == 0) {
         vars[k] = -(vars[k] + 1);
@@ -521,8 +564,8 @@ __global__ __launch_bounds__(NT) void level_kernel(
     const int* __restrict__ big_mask, int lds_doubles, double ustab,
     double pivot_tol, int chol, int dcap_big, const double* __restrict__ sc) {
   extern __shared__ double smem[];  // F (lds_doubles) then the cached column
-  __shared__ VI red[NT / 64];
-  __shared__ ColStats cred[NT / 64];
+  __shared__ VI red[2 * NT / 64];
+  __shared__ VI cred[2 * NT / 64];
   __shared__ int sh[4];
   __shared__ int cnt[4];
   const int tid = threadIdx.x;
@@ -779,6 +822,297 @@ __device__ __forceinline__ double panel_dot(double s, const double* __restrict__
   return (s + s1) + (s2 + s3);
 }
 
+// LDS-resident panel (fronts with at most NTP rows from the panel's first
+// pivot down): thread t owns row kstart + t, and the panel's columns
+// [kstart, kstart + NBR + 2) of all rows >= kstart live in LDS, column-major
+// (Ls[q * NTP + t]): not yet updated columns, overwritten by L as they are
+// factored.  A column update is conflict-free LDS reads against row k of
+// W = L D (formed from the L rows and the D blocks) broadcast through LDS;
+// the imax candidate column and symmetric interchanges inside the panel
+// stay in LDS; the block goes back to global memory at the end.  Pivot
+// rules and the W contents are those of the global-memory loop below.
+constexpr int LS_COLS = NBR + 2;
+constexpr size_t PANEL_LDS_BYTES = sizeof(double) * LS_COLS * NTP;
+
+struct PanelShared {
+  VI red[2 * NTP / 64];
+  double wrow[NBR + 2];
+  double wrow2[NBR + 2];
+  double dd[NBR + 2];   // D diagonal of panel column q
+  double de[NBR + 2];   // D sub-diagonal of a 2x2 block starting at q
+  int two[NBR + 2];     // 0: 1x1, 1: first column of a 2x2, 2: second
+  double xch[4];
+  double sval[4];
+  int cnt[4];
+};
+
+// Entry q of row t of W = L D (1x1 and 2x2 blocks of D).
+__device__ __forceinline__ double ld_entry(const double* Ls, int t, int q, const PanelShared& sh) {
+  const double l = Ls[q * NTP + t];
+  if (sh.two[q] == 0) return l * sh.dd[q];
+  if (sh.two[q] == 1) return l * sh.dd[q] + Ls[(q + 1) * NTP + t] * sh.de[q];
+  return Ls[(q - 1) * NTP + t] * sh.de[q - 1] + l * sh.dd[q];
+}
+
+__device__ __forceinline__ double lds_dot(double s, const double* Ls, int t, const double* w,
+                                          int kw) {
+  double s1 = 0.0;
+  int q = 0;
+  for (; q + 2 <= kw; q += 2) {
+    s -= Ls[q * NTP + t] * w[q];
+    s1 -= Ls[(q + 1) * NTP + t] * w[q + 1];
+  }
+  if (q < kw) s -= Ls[q * NTP + t] * w[q];
+  return s + s1;
+}
+
+// Exchange the current column values s, s2 of the owners of rows a, b
+// (owner-relative ta, tb).  Ends with a barrier.
+__device__ __forceinline__ void swap_lds_rows_values(double& s, double& s2, int tid, int ta,
+                                                     int tb, PanelShared& sh) {
+  if (tid == ta) {
+    sh.xch[0] = s;
+    sh.xch[1] = s2;
+  }
+  if (tid == tb) {
+    sh.xch[2] = s;
+    sh.xch[3] = s2;
+  }
+  __syncthreads();
+  if (tid == ta) {
+    s = sh.xch[2];
+    s2 = sh.xch[3];
+  }
+  if (tid == tb) {
+    s = sh.xch[0];
+    s2 = sh.xch[1];
+  }
+  __syncthreads();
+}
+
+// Panel block <-> global front: columns [kstart, kstart + pw), rows >= the
+// column (lower triangle).
+__device__ __forceinline__ void panel_load(const double* F, int ld, int m, int kstart, int pw,
+                                           double* Ls) {
+  const int t = threadIdx.x, rows = m - kstart;
+  if (t < rows)
+    for (int q = 0; q < pw; q++)
+      Ls[q * NTP + t] = t >= q ? F[kstart + t + (size_t)(kstart + q) * ld] : 0.0;
+}
+__device__ __forceinline__ void panel_store(double* F, int ld, int m, int kstart, int pw,
+                                            const double* Ls) {
+  const int t = threadIdx.x, rows = m - kstart;
+  if (t < rows)
+    for (int q = 0; q < pw && q <= t; q++)
+      F[kstart + t + (size_t)(kstart + q) * ld] = Ls[q * NTP + t];
+}
+
+// Symmetric interchange a < b of the front when both columns lie in the
+// LDS panel (sparse_ldl.h:1252-1274 restricted to the stored lower
+// triangle): rows a, b of the earlier panels' columns are swapped in global
+// memory, everything else in LDS.  Ends with a barrier.
+__device__ void panel_symm_swap(double* F, int ld, int m, int kstart, int* vars, int a, int b,
+                                double* Ls) {
+  const int t0 = threadIdx.x;
+  const int qa = a - kstart, qb = b - kstart;
+  // Row part, columns c < a.
+  for (int c = t0; c < a; c += NTP) {
+    if (c < kstart)
+      swapd(F[a + (size_t)c * ld], F[b + (size_t)c * ld]);
+    else
+      swapd(Ls[(c - kstart) * NTP + qa], Ls[(c - kstart) * NTP + qb]);
+  }
+  if (t0 == 0) {
+    swapd(Ls[qa * NTP + qa], Ls[qb * NTP + qb]);
+    const int v = vars[a];
+    vars[a] = vars[b];
+    vars[b] = v;
+  }
+  // a < t < b: F[t, a] <-> F[b, t].
+  for (int t = a + 1 + t0; t < b; t += NTP)
+    swapd(Ls[qa * NTP + (t - kstart)], Ls[(t - kstart) * NTP + qb]);
+  // t > b: F[t, a] <-> F[t, b].
+  for (int t = b + 1 + t0; t < m; t += NTP)
+    swapd(Ls[qa * NTP + (t - kstart)], Ls[qb * NTP + (t - kstart)]);
+  __syncthreads();
+}
+
+__device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& nc, int& kw,
+                          int kstart, bool bk, bool chol, double ustab, double pivot_tol,
+                          double* Ls, PanelShared& sh) {
+  const int ld = m, ldw = m;
+  const int tid = threadIdx.x;
+  const int r = kstart + tid;
+  const bool own = r < m;
+  const int pw = min(LS_COLS, m - kstart);
+  panel_load(F, ld, m, kstart, pw, Ls);
+  __syncthreads();
+  // Interchange of a < b, in LDS when both columns are in the panel,
+  // otherwise through global memory (panel written back and reloaded).
+  auto interchange = [&](int a, int b, int wcols, double& s, double& s2) {
+    if (b - kstart < pw) {
+      panel_symm_swap(F, ld, m, kstart, vars, a, b, Ls);
+    } else {
+      panel_store(F, ld, m, kstart, pw, Ls);
+      __syncthreads();
+      symm_swap_t<NTP>(F, ld, m, vars, a, b);
+      panel_load(F, ld, m, kstart, pw, Ls);
+    }
+    swap_w_rows_t<NTP>(W, ldw, a, b, wcols);
+    swap_lds_rows_values(s, s2, tid, a - kstart, b - kstart, sh);
+  };
+  while (kw < NBR && k < nc) {
+    // Row k of W = L D, one entry per thread.
+    if (tid < kw) sh.wrow[tid] = ld_entry(Ls, k - kstart, tid, sh);
+    __syncthreads();
+    double s = 0.0, s2 = 0.0;
+    VI a = {-1.0, INT_MAX}, b = {-1.0, INT_MAX};
+    if (own && r >= k) {
+      s = lds_dot(Ls[kw * NTP + tid], Ls, tid, sh.wrow, kw);
+      W[r + (size_t)kw * ldw] = s;
+      if (r == k) sh.sval[0] = s;
+      if (r > k) {
+        const VI c = {fabs(s), r};
+        if (r < nc)
+          a = c;
+        else
+          b = c;
+      }
+    }
+    int step = 1, kpiv = k;
+    bool zero = false;
+    if (chol) {
+      __syncthreads();
+    } else {
+      block_argmax2_dpp<NTP>(a, b, sh.red);
+      const double akk = fabs(sh.sval[0]);
+      const bool has_cand = k + 1 < nc && a.i > k && a.i < nc;
+      const int imax = has_cand ? a.i : 0;
+      const double abswik = has_cand ? a.v : 0.0;
+      const double colmax = bk ? abswik : fmax(abswik, nc < m ? b.v : 0.0);
+      const bool ok1 = bk ? (fmax(akk, colmax) == 0.0 || akk >= BK_ALPHA * colmax)
+                          : (akk >= pivot_tol && akk >= ustab * colmax);
+      if (ok1) {
+        zero = bk && fmax(akk, colmax) == 0.0;
+      } else if (has_cand) {
+        if (tid < kw) sh.wrow2[tid] = ld_entry(Ls, imax - kstart, tid, sh);
+        // Not yet updated column imax, rows >= k: rows below imax from its
+        // column, rows k..imax-1 from row imax (the columns r < imax).
+        double fi = 0.0;
+        if (own && r >= k) {
+          const int qi = imax - kstart;
+          if (r >= imax)
+            fi = qi < pw ? Ls[qi * NTP + tid] : F[r + (size_t)imax * ld];
+          else
+            fi = tid < pw ? Ls[tid * NTP + qi] : F[imax + (size_t)r * ld];
+        }
+        __syncthreads();
+        VI rr = {-1.0, INT_MAX}, dummy = {-1.0, INT_MAX};
+        if (own && r >= k) {
+          s2 = lds_dot(fi, Ls, tid, sh.wrow2, kw);
+          W[r + (size_t)(kw + 1) * ldw] = s2;
+          if (r == imax) {
+            sh.sval[1] = s2;
+            sh.sval[2] = s;
+          } else {
+            rr = VI{fabs(s2), r};
+          }
+        }
+        block_argmax2_dpp<NTP>(rr, dummy, sh.red);
+        const double rowmax = rr.v;
+        const double aii = fabs(sh.sval[1]);
+        if (bk) {
+          if (akk >= BK_ALPHA * colmax * (colmax / rowmax)) {
+            step = 1;
+            kpiv = k;
+          } else if (aii >= BK_ALPHA * rowmax) {
+            step = 1;
+            kpiv = imax;
+          } else {
+            step = 2;
+            kpiv = imax;
+          }
+        } else if (aii >= pivot_tol && aii >= ustab * rowmax) {
+          step = 1;
+          kpiv = imax;
+        } else {
+          const double wkk = sh.sval[0], wii = sh.sval[1], wik = sh.sval[2];
+          const double det = fabs(wkk * wii - wik * wik);
+          step = 0;
+          if (det >= pivot_tol) {
+            const double delta = det / fmax(aii + abswik, akk + abswik);
+            if (delta >= ustab * colmax && delta >= ustab * rowmax) {
+              step = 2;
+              kpiv = imax;
+            }
+          }
+        }
+        if (step == 1 && kpiv == imax && own && r >= k) {
+          s = s2;
+          W[r + (size_t)kw * ldw] = s2;
+        }
+      } else {
+        step = 0;
+      }
+    }
+
+    if (step == 0) {
+      // The candidate goes to the end of the candidates (sparse_ldl.h:1441).
+      __syncthreads();
+      if (k < nc - 1) interchange(k, nc - 1, kw, s, s2);
+      nc -= 1;
+      continue;
+    }
+    const int ksrc = k + step - 1;
+    if (kpiv != ksrc) {
+      __syncthreads();
+      interchange(ksrc, kpiv, kw + step, s, s2);
+    }
+    const int q = kw;
+    if (step == 1) {
+      const double d = kpiv == k ? sh.sval[0] : sh.sval[1];
+      if (own && r >= k) {
+        double v;
+        if (r == k)
+          v = d;
+        else
+          v = zero ? 0.0 : s * (1.0 / d);
+        Ls[q * NTP + tid] = v;
+      }
+      if (tid == 0) {
+        sh.dd[q] = d;
+        sh.two[q] = 0;
+        count_inertia(1, d, 0.0, 0.0, zero, sh.cnt);
+      }
+    } else {
+      const double d11 = sh.sval[0], d21 = sh.sval[2], d22 = sh.sval[1];
+      const double detinv = 1.0 / (d11 * d22 - d21 * d21);
+      const double i00 = d22 * detinv, i01 = -d21 * detinv, i11 = d11 * detinv;
+      if (own && r >= k + 2) {
+        Ls[q * NTP + tid] = s * i00 + s2 * i01;
+        Ls[(q + 1) * NTP + tid] = s * i01 + s2 * i11;
+      }
+      if (tid == 0) {
+        Ls[q * NTP + q] = d11;
+        Ls[q * NTP + q + 1] = d21;
+        Ls[(q + 1) * NTP + q + 1] = d22;
+        vars[k] = -(vars[k] + 1);
+        vars[k + 1] = -(vars[k + 1] + 1);
+        sh.dd[q] = d11;
+        sh.de[q] = d21;
+        sh.dd[q + 1] = d22;
+        sh.two[q] = 1;
+        sh.two[q + 1] = 2;
+        count_inertia(2, d11, d21, d22, false, sh.cnt);
+      }
+    }
+    k += step;
+    kw += step;
+    __syncthreads();
+  }
+  panel_store(F, ld, m, kstart, pw, Ls);
+}
+
 // ---------------------------------------------------------------------------
 // Blocked path for large fronts.  One panel: up to NB pivots selected with
 // the reference's left-looking panel loop (sparse_ldl.h:1309-1505); W holds
@@ -788,6 +1122,7 @@ __device__ __forceinline__ double panel_dot(double s, const double* __restrict__
 // update W[k:, kw] = F[k:, k] - F[k:, kstart:k] W[k, :kw]^T reads row k of W
 // from LDS, and the candidate / contribution maxima come out of the same
 // pass with one fused reduction.
+template <bool REG>
 __global__ __launch_bounds__(NTP) void panel_kernel(DevState S,
                                                     const int* __restrict__ bnodes,
                                                     const int* __restrict__ broot,
@@ -801,8 +1136,9 @@ __global__ __launch_bounds__(NTP) void panel_kernel(DevState S,
   __shared__ VI red[2 * NTP / 64];
   __shared__ double wrow[NBW];
   __shared__ double sval[2];
-  __shared__ int cnt[4];
+  __shared__ PanelShared psh;
   __shared__ int bad;
+  int* cnt = psh.cnt;
   const int tid = threadIdx.x;
   if (tid == 0) bad = atomicAdd(&S.flags[0], 0);
   __syncthreads();
@@ -817,7 +1153,11 @@ __global__ __launch_bounds__(NTP) void panel_kernel(DevState S,
   if (tid == 0) cnt[0] = cnt[1] = cnt[2] = cnt[3] = 0;
   __syncthreads();
   if (k >= nc) {
-    if (tid == 0) {
+    // Finished in an earlier step: make the following trailing launch a
+    // no-op.  Only the first of the two instantiations launched per step
+    // (REG) does this; the second may see a front the first just finished,
+    // whose trailing update is still to come.
+    if (REG && tid == 0) {
       ps[2] = k;
       ps[3] = 0;
     }
@@ -826,6 +1166,14 @@ __global__ __launch_bounds__(NTP) void panel_kernel(DevState S,
   const int kstart = k;
   int kw = 0;
   const bool bk = root != 0;
+  // Fronts whose panel rows fit one row per thread take the LDS path; the
+  // two paths are separate instantiations so that each gets its own
+  // register allocation, and each returns on the other's fronts.
+  if ((m - kstart <= NTP) != REG) return;
+  if constexpr (REG) {
+    extern __shared__ double Ls[];
+    panel_lds(F, W, vars, m, k, nc, kw, kstart, bk, chol != 0, ustab, pivot_tol, Ls, psh);
+  } else {
   while (kw < NB && k < nc) {
     double* wk = W + (size_t)kw * ldw;
     double* wi = W + (size_t)(kw + 1) * ldw;
@@ -961,6 +1309,7 @@ __global__ __launch_bounds__(NTP) void panel_kernel(DevState S,
     __syncthreads();
     k += step;
     kw += step;
+  }
   }
   if (tid == 0) {
     ps[0] = k;
@@ -1458,6 +1807,7 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   big_npanels_.assign(nlev, 0);
   big_tiles_.assign(nlev, 0);
   big_maxkids_.assign(nlev, 0);
+  big_wide_.assign(nlev, 0);
   h_kptr_ = plan.children_ptr;
   std::vector<int> big_root, big_ns;
   std::vector<int64_t> big_woff;
@@ -1474,9 +1824,10 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
         const int mcap = ns[k] + cs[k] + DCAP_BIG;
         big_woff.push_back(wtot);
         wtot += (int64_t)mcap * NBW;
-        big_npanels_[l] = std::max(big_npanels_[l], (ns[k] + DCAP_BIG + NB - 1) / NB);
+        big_npanels_[l] = std::max(big_npanels_[l], (ns[k] + DPAN_BIG + NBR - 1) / NBR);
         big_maxkids_[l] = std::max(big_maxkids_[l],
                                    plan.children_ptr[k + 1] - plan.children_ptr[k]);
+        if (mcap > NTP) big_wide_[l] = 1;
         big_tiles_[l] = std::max(big_tiles_[l], (mcap + TILE - 1) / TILE);
       }
       mx = std::max(mx, std::min(ns[k] + cs[k] + 32, LDS_MAX_M));
@@ -1553,7 +1904,10 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
     if (!attr_done) {
       HIPCHK(hipFuncSetAttribute((const void*)level_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (LDS_MAX_M * LDS_MAX_M + LDS_MAX_M) * (int)sizeof(double)));
+                                 (LDS_MAX_M * LDS_MAX_M + 2 * LDS_MAX_M) * (int)sizeof(double)));
+      HIPCHK(hipFuncSetAttribute((const void*)panel_kernel<true>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)PANEL_LDS_BYTES));
       HIPCHK(hipFuncSetAttribute((const void*)solve_fwd_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  150 * 1024));
@@ -1616,7 +1970,7 @@ int LdlNumeric::run_factor(const double* vals, const double* shift,
     const int lds = level_lds_[l];
     tk_begin(stream);
     hipLaunchKernelGGL(level_kernel, dim3(cnt), dim3(NT),
-                       size_t(lds + LDS_MAX_M) * sizeof(double), stream, P_, S_, vals,
+                       size_t(lds + 2 * LDS_MAX_M) * sizeof(double), stream, P_, S_, vals,
                        shift, d_level_nodes_ + level_ptr_[l], d_big_mask_, lds,
                        prm.ustab, prm.pivot_tol, cholesky ? 1 : 0,
                        sync_mode ? -1 : DCAP_BIG,
@@ -1656,7 +2010,7 @@ int LdlNumeric::run_factor(const double* vals, const double* shift,
         lns.push_back(h_ns_[node]);
         wo.push_back(tot);
         tot += (int64_t)hm[node] * NBW;
-        npanels = std::max(npanels, (h_ns_[node] + hD[node] + NB - 1) / NB);
+        npanels = std::max(npanels, (h_ns_[node] + hD[node] + NBR - 1) / NBR);
         tiles = std::max(tiles, (hm[node] + TILE - 1) / TILE);
       }
       nlaunch = int(ln.size());
@@ -1702,9 +2056,13 @@ int LdlNumeric::run_factor(const double* vals, const double* shift,
     }
     for (int q = 0; q < npanels; q++) {
       tk_begin(stream);
-      hipLaunchKernelGGL(panel_kernel, dim3(nlaunch), dim3(NTP), 0, stream, S_,
+      hipLaunchKernelGGL(panel_kernel<true>, dim3(nlaunch), dim3(NTP), PANEL_LDS_BYTES, stream, S_,
                          bnodes, broot, W_, woff, prm.ustab, prm.pivot_tol,
                          cholesky ? 1 : 0);
+      if (big_wide_[l] || sync_mode)
+        hipLaunchKernelGGL(panel_kernel<false>, dim3(nlaunch), dim3(NTP), 0, stream, S_,
+                           bnodes, broot, W_, woff, prm.ustab, prm.pivot_tol,
+                           cholesky ? 1 : 0);
       tk_end(stream, TK_PANEL);
       tk_begin(stream);
       hipLaunchKernelGGL(trailing_kernel, dim3(tiles, tiles, nlaunch), dim3(NT), 0,
