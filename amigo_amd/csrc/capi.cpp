@@ -227,7 +227,7 @@ int amigo_ldl_factor_shifted(amigo_ldl* h, const double* values_dev,
   std::string err;
   int rc = h->num.factor(values_dev, shift_dev, h->ustab, h->pivot_tol,
                          h->solver_type == AMIGO_SOLVER_CHOLESKY,
-                         static_cast<hipStream_t>(stream), err);
+                         precision == AMIGO_PREC_MIXED, static_cast<hipStream_t>(stream), err);
   if (rc != 0) return fail(rc, "amigo_ldl_factor: " + err);
   return AMIGO_OK;
 }

@@ -50,6 +50,7 @@ struct DevState {
 struct FactorParams {
   double ustab;
   double pivot_tol;
+  bool mixed;  // Schur updates of the blocked fronts in FP32 (MFMA)
 };
 
 // Kernel classes timed with HIP events when timing is on.
@@ -71,7 +72,7 @@ class LdlNumeric {
   int setup(const SymbolicPlan& plan, const int* rowp, const int* cols,
             double delay_growth, std::string& err);
   int factor(const double* vals, const double* shift, double ustab,
-             double pivot_tol, bool cholesky, hipStream_t stream,
+             double pivot_tol, bool cholesky, bool mixed, hipStream_t stream,
              std::string& err);
   int solve(int nrhs, double* x, int ldx, hipStream_t stream,
             std::string& err);

@@ -1380,6 +1380,90 @@ __global__ __launch_bounds__(NT) void trailing_kernel(DevState S,
   }
 }
 
+// Trailing update on MFMA: F[k:, k:] -= F[k:, kstart:kstart+kw] W[k:, :kw]^T
+// on the lower triangle, 64x64 tiles staged through LDS in precision T
+// (double: v_mfma_f64_16x16x4f64; float, the mixed-precision form:
+// v_mfma_f32_16x16x4f32 with FP32 accumulation, result subtracted from the
+// FP64 front).  Four waves per tile, each a 32x32 quadrant of 2x2 MFMA
+// 16x16 blocks; K padded to a multiple of 4 with zeros.
+// Operand layout of the 16x16x4 MFMAs: lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16].  Result register j of lane l holds
+// D[4 (l / 16) + j][l % 16] for f32 and D[l / 16 + 4 j][l % 16] for f64
+// (checked on gfx950).
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ mfma_f4 mfma_16x16x4(float a, float b, mfma_f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ mfma_d4 mfma_16x16x4(double a, double b, mfma_d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+constexpr int TK_PAD = ((NBW + 3) / 4) * 4;  // padded K of a panel
+
+template <typename T, typename V4>
+__global__ __launch_bounds__(NT) void trailing_mfma_kernel(DevState S,
+                                                           const int* __restrict__ bnodes,
+                                                           const double* __restrict__ Wbase,
+                                                           const int64_t* __restrict__ woff) {
+  const int node = bnodes[blockIdx.z];
+  const double* W = Wbase + woff[blockIdx.z];
+  __shared__ T As[TILE][TK_PAD + 1];
+  __shared__ T Bs[TILE][TK_PAD + 1];
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = atomicAdd(&S.flags[0], 0);
+  __syncthreads();
+  if (bad != 0) return;
+  const int* ps = S.pstate + PS * node;
+  const int m = S.m[node];
+  const int k = ps[0], kstart = ps[2], kw = ps[3];
+  const int ti = blockIdx.x, tj = blockIdx.y;
+  if (kw == 0 || ti < tj) return;
+  const int i0 = k + ti * TILE, j0 = k + tj * TILE;
+  if (i0 >= m || j0 >= m) return;
+  double* F = S.rpool + S.foff[node];
+  const int ld = m, ldw = m;
+  const int tid = threadIdx.x;
+  const int kp = (kw + 3) & ~3;
+  for (int e = tid; e < TILE * kp; e += NT) {
+    const int r = e % TILE, q = e / TILE;
+    const int i = i0 + r, j = j0 + r;
+    As[r][q] = (i < m && q < kw) ? (T)F[i + (size_t)(kstart + q) * ld] : (T)0;
+    Bs[r][q] = (j < m && q < kw) ? (T)W[j + (size_t)q * ldw] : (T)0;
+  }
+  __syncthreads();
+  const int lane = tid & 63, wv = tid >> 6;
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  V4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = V4{0, 0, 0, 0};
+  for (int q = 0; q < kp; q += 4) {
+    T av[2], bv[2];
+#pragma unroll
+    for (int a = 0; a < 2; a++) av[a] = As[wr + a * 16 + lr][q + lk];
+#pragma unroll
+    for (int b = 0; b < 2; b++) bv[b] = Bs[wc + b * 16 + lr][q + lk];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) acc[a][b] = mfma_16x16x4(av[a], bv[b], acc[a][b]);
+  }
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) {
+        const int i = i0 + wr + a * 16 + (sizeof(T) == 8 ? lk + 4 * jj : lk * 4 + jj);
+        const int j = j0 + wc + b * 16 + lr;
+        if (i < m && j < m && i >= j) F[i + (size_t)j * ld] -= (double)acc[a][b][jj];
+      }
+}
+
 // Close the blocked fronts of a level: pivot/delay counts, and a check that
 // the statically sized panel sequence finished the front (otherwise the
 // factorization is redone level-synchronously).
@@ -1477,6 +1561,85 @@ __global__ __launch_bounds__(NT) void fpos_kernel(DevPlan P, DevState S, int K) 
 }
 
 // ---------------------------------------------------------------------------
+// Wave-synchronous pivot-block solves (p <= 64 pivots): lane l holds t[l],
+// the p x p pivot block L11 (with the D blocks on its diagonal) is staged in
+// LDS column-major, and each pivot step broadcasts one (two) value(s) with
+// readlane instead of a workgroup barrier.
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Stage L[0:p, 0:p] (ld) into Ls (ld p); all threads, ends with a barrier.
+__device__ __forceinline__ void stage_pivot_block(const double* L, int ld, int p, double* Ls) {
+  for (int e = threadIdx.x; e < p * p; e += NT) {
+    const int jj = e / p, ii = e - jj * p;
+    Ls[e] = L[ii + (size_t)jj * ld];
+  }
+  __syncthreads();
+}
+
+// Forward: unit-lower solve of t[0:p) (in place, lane l <- t[l]) and the
+// roles of the lanes (0: 1x1, 1: first of a 2x2, 2: second).  Wave 0 only.
+__device__ __forceinline__ double wave_lower_solve(double tl, int l, int p, const int* vars,
+                                                   const double* Ls, int& role) {
+  const int vl = l < p ? vars[l] : 0;
+  role = 0;
+  for (int j = 0; j < p;) {
+    const bool two = __builtin_amdgcn_readlane(vl, j) < 0;
+    if (!two) {
+      const double tj = readlane_d(tl, j);
+      if (l > j && l < p) tl -= Ls[l + j * p] * tj;
+      j += 1;
+    } else {
+      const double t1 = readlane_d(tl, j), t2 = readlane_d(tl, j + 1);
+      if (l > j + 1 && l < p) tl -= Ls[l + j * p] * t1 + Ls[l + (j + 1) * p] * t2;
+      if (l == j) role = 1;
+      if (l == j + 1) role = 2;
+      j += 2;
+    }
+  }
+  return tl;
+}
+
+// D solve of the lane's pivot (1x1, or its half of a 2x2 block).
+__device__ __forceinline__ double wave_d_solve(double tl, int l, int p, int role,
+                                               const double* Ls) {
+  const double up = __shfl(tl, min(l + 1, 63), 64);
+  const double dn = __shfl(tl, max(l - 1, 0), 64);
+  if (l >= p) return tl;
+  if (role == 0) return tl / Ls[l + l * p];
+  if (role == 1) {
+    const double d11 = Ls[l + l * p], d21 = Ls[l + 1 + l * p], d22 = Ls[l + 1 + (l + 1) * p];
+    const double inv = 1.0 / (d11 * d22 - d21 * d21);
+    return inv * (d22 * tl - d21 * up);
+  }
+  const double d11 = Ls[l - 1 + (l - 1) * p], d21 = Ls[l + (l - 1) * p], d22 = Ls[l + l * p];
+  const double inv = 1.0 / (d11 * d22 - d21 * d21);
+  return inv * (d11 * tl - d21 * dn);
+}
+
+// Backward: unit-upper (L11^T) solve of t[0:p), wave 0 only.
+__device__ __forceinline__ double wave_upper_solve(double tl, int l, int p, const int* vars,
+                                                   const double* Ls) {
+  const int vl = l < p ? vars[l] : 0;
+  for (int i = p - 1; i >= 0;) {
+    if (__builtin_amdgcn_readlane(vl, i) >= 0) {
+      const double ti = readlane_d(tl, i);
+      if (l < i) tl -= Ls[i + l * p] * ti;
+      i -= 1;
+    } else {
+      const double t2 = readlane_d(tl, i), t1 = readlane_d(tl, i - 1);
+      if (l < i - 1) tl -= Ls[i + l * p] * t2 + Ls[i - 1 + l * p] * t1;
+      i -= 2;
+    }
+  }
+  return tl;
+}
+
+// ---------------------------------------------------------------------------
 // Level-scheduled solves (sparse_ldl.h:1807-1908).  Forward, multifrontal
 // form: the front vector t holds the right-hand side on the pivot rows plus
 // the children's update vectors (extend-added in child order, so the sums
@@ -1519,6 +1682,32 @@ __global__ __launch_bounds__(NT) void solve_fwd_kernel(DevPlan P, DevState S,
       }
       doff += dc;
       __syncthreads();
+    }
+    if (p <= 64) {
+      double* Ls = t + m;
+      stage_pivot_block(L, ld, p, Ls);
+      double tl = 0.0;
+      int role = 0;
+      if (tid < 64) {
+        tl = wave_lower_solve(tid < p ? t[tid] : 0.0, tid, p, vars, Ls, role);
+        if (tid < p) t[tid] = tl;
+      }
+      __syncthreads();
+      double* uo = ur + S.voff[node];
+      for (int i = p + tid; i < m; i += NT) {
+        double s = t[i];
+        for (int j = 0; j < p; j++) s -= L[i + (size_t)j * ld] * t[j];
+        uo[i] = s;
+      }
+      if (tid < 64) {
+        tl = wave_d_solve(tl, tid, p, role, Ls);
+        if (tid < p) {
+          const int v = vars[tid];
+          xr[v >= 0 ? v : -v - 1] = tl;
+        }
+      }
+      __syncthreads();
+      continue;
     }
     for (int j = 0; j < p;) {
       if (vars[j] >= 0) {
@@ -1594,6 +1783,19 @@ __global__ __launch_bounds__(NT) void solve_bwd_kernel(DevState S,
       if (lane == 0) t[j] -= s;
     }
     __syncthreads();
+    if (p <= 64) {
+      double* Ls = t + m;
+      stage_pivot_block(L, ld, p, Ls);
+      if (tid < 64) {
+        const double tl = wave_upper_solve(tid < p ? t[tid] : 0.0, tid, p, vars, Ls);
+        if (tid < p) {
+          const int v = vars[tid];
+          xr[v >= 0 ? v : -v - 1] = tl;
+        }
+      }
+      __syncthreads();
+      continue;
+    }
     for (int i = p - 1; i >= 0;) {
       if (vars[i] >= 0) {
         const double ti = t[i];
@@ -1955,6 +2157,7 @@ int LdlNumeric::ensure_pools(unsigned long long rcap, unsigned long long icap,
 int LdlNumeric::run_factor(const double* vals, const double* shift,
                            FactorParams prm, bool cholesky, bool sync_mode,
                            hipStream_t stream, std::string& err) {
+  const bool mixed = prm.mixed;
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, stream, S_);
   HIPCHK(hipGetLastError());
   if (scaling_) {
@@ -2065,8 +2268,12 @@ int LdlNumeric::run_factor(const double* vals, const double* shift,
                            cholesky ? 1 : 0);
       tk_end(stream, TK_PANEL);
       tk_begin(stream);
-      hipLaunchKernelGGL(trailing_kernel, dim3(tiles, tiles, nlaunch), dim3(NT), 0,
-                         stream, S_, bnodes, (const double*)W_, woff);
+      if (mixed)
+        hipLaunchKernelGGL((trailing_mfma_kernel<float, mfma_f4>), dim3(tiles, tiles, nlaunch),
+                           dim3(NT), 0, stream, S_, bnodes, (const double*)W_, woff);
+      else
+        hipLaunchKernelGGL((trailing_mfma_kernel<double, mfma_d4>), dim3(tiles, tiles, nlaunch),
+                           dim3(NT), 0, stream, S_, bnodes, (const double*)W_, woff);
       tk_end(stream, TK_TRAILING);
     }
     hipLaunchKernelGGL(finalize_big_kernel, dim3((nlaunch + 63) / 64), dim3(64), 0,
@@ -2093,9 +2300,9 @@ int LdlNumeric::run_factor(const double* vals, const double* shift,
 }
 
 int LdlNumeric::factor(const double* vals, const double* shift, double ustab,
-                       double pivot_tol, bool cholesky, hipStream_t stream,
+                       double pivot_tol, bool cholesky, bool mixed, hipStream_t stream,
                        std::string& err) {
-  FactorParams prm{ustab, pivot_tol};
+  FactorParams prm{ustab, pivot_tol, mixed};
   factored_ = false;
   bool sync_mode = false;
   for (int attempt = 0; attempt < 8; attempt++) {
@@ -2138,7 +2345,7 @@ int LdlNumeric::solve(int nrhs, double* x, int ldx, hipStream_t stream,
     err = "solve called before a successful factor";
     return -3;
   }
-  const size_t lds = size_t(max_p_) * sizeof(double);
+  const size_t lds = size_t(max_p_ + 64 * 64) * sizeof(double);
   if (lds > 160 * 1024) {
     err = "front too large for the LDS solve kernels";
     return -1;

@@ -49,6 +49,8 @@ class AmigoSolver:
         # Optionally keep the first factorized matrix (values, shift).
         self.keep_first = False
         self.first_values = None
+        self.fp64_fallbacks = 0
+        self.ir_steps_total = 0
 
     # ---- assembly + factorization ------------------------------------------
     def _factor_current(self):
@@ -103,7 +105,25 @@ class AmigoSolver:
                              backend, max_steps=10):
         """FP64 refinement on the unreduced 8-block system
         (linear_solver.py:109-264).  The stopping rule is the reference's:
-        ratio < 1e-10 after at least one step, or a stall, at most 10."""
+        ratio < 1e-10 after at least one step, or a stall, at most 10.
+
+        With the mixed-precision factor (FP32 Schur updates), refinement
+        that ends above the 1e-10 ratio falls back to an FP64 factorization
+        of the same matrix and refines again from a fresh solve."""
+        steps, ratio = self._refine(mu, rhs_saved, vars_, grad, px, res, corr, backend, max_steps)
+        if self.ldl.precision == "mixed" and not ratio < 1e-10:
+            # The FP64 factor then serves the rest of this iteration's solves.
+            self.ldl.factor(shift=self.shift, precision="fp64")
+            self.n_factor += 1
+            self.fp64_fallbacks += 1
+            self.solve(rhs_saved, px)
+            s2, ratio = self._refine(mu, rhs_saved, vars_, grad, px, res, corr, backend, max_steps)
+            steps += s2
+        self.ir_steps_total += steps
+        self.last_ir = (steps, ratio)
+        return steps, ratio
+
+    def _refine(self, mu, rhs_saved, vars_, grad, px, res, corr, backend, max_steps):
         L = _lib.lib()
         info = backend.info
         st = vars_.state()
@@ -154,5 +174,4 @@ class AmigoSolver:
                 "ipm",
             )
             steps += 1
-        self.last_ir = (steps, ratio)
         return steps, ratio

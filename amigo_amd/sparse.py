@@ -124,6 +124,8 @@ class SparseLDL:
         self.mat = mat
         self.n = mat.nrows
         self.solver_type = CHOLESKY if str(solver_type).upper().endswith("CHOLESKY") else LDL
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
         h = ctypes.c_void_p()
         gp = None
@@ -180,6 +182,8 @@ class SparseLDL:
     def factor(self, shift=None, precision=None):
         """Numeric factorization of mat.data (+ optional diagonal shift)."""
         prec = precision or self.precision
+        if prec not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
         code = 1 if prec == "mixed" else 0
         vals = self.mat.data
         if shift is None:
