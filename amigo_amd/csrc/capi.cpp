@@ -323,4 +323,26 @@ int amigo_ldl_work(const amigo_ldl* h, double* factor_flops,
   return AMIGO_OK;
 }
 
+int amigo_ldl_memory(const amigo_ldl* h, long long* bytes) {
+  if (!h || !bytes) return fail(AMIGO_ERR_ARG, "null argument");
+  // Device bytes the numeric phase allocates for this plan: front pools
+  // (delay_growth x static sizes, the sizing rule of LdlNumeric::setup), the
+  // index / position pools, the forward-solve update vectors (one RHS), the
+  // panel workspace of the widest level, and the static plan arrays.
+  const SymbolicPlan& P = h->plan;
+  const double g = std::max(1.0, h->delay_growth);
+  double r = 0.0, iv = 0.0;
+  for (int k = 0; k < P.num_nodes; k++) {
+    const double m = (P.node_ptr[k + 1] - P.node_ptr[k]) + (P.contrib_ptr[k + 1] - P.contrib_ptr[k]);
+    r += m * m;
+    iv += m;
+  }
+  const double rcap = g * r + 4096.0, icap = g * iv + 4096.0;
+  double total = 8.0 * rcap + 2.0 * 4.0 * icap + 8.0 * icap;
+  total += 8.0 * 68.0 * (iv + 64.0 * P.num_nodes);          // panel workspace bound
+  total += 4.0 * (6.0 * P.num_nodes + 3.0 * P.contrib_rows.size() + 4.0 * h->nnz + 2.0 * h->n);
+  *bytes = (long long)total;
+  return AMIGO_OK;
+}
+
 }  // extern "C"

@@ -2326,12 +2326,9 @@ int LdlNumeric::factor(const double* vals, const double* shift, double ustab,
       if (err.empty()) err = "factorization failed";
       return rc;
     }
-    // Max pivot block per level for the solve kernels' LDS sizing.
-    std::vector<int> np(K_), mm(K_);
-    HIPCHK(hipMemcpy(np.data(), S_.npiv, K_ * sizeof(int), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(mm.data(), S_.m, K_ * sizeof(int), hipMemcpyDeviceToHost));
-    max_p_ = 0;
-    for (int k = 0; k < K_; k++) max_p_ = std::max(max_p_, mm[k]);
+    // Largest front of this factorization (reduced on the device): the
+    // solve kernels' LDS sizing.
+    max_p_ = max_front_seen_;
     factored_ = true;
     return 0;
   }

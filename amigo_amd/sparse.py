@@ -262,6 +262,12 @@ class SparseLDL:
         names = ["level", "panel", "schur_update", "solve_fwd", "solve_bwd", "assemble_big"]
         return {k: (ms[i], cnt[i]) for i, k in enumerate(names)}, (fs.value, fb.value)
 
+    def memory_bytes(self):
+        """Predicted device bytes of the numeric phase (pools, workspaces)."""
+        b = ctypes.c_longlong()
+        _lib.check(_lib.lib().amigo_ldl_memory(self._h, ctypes.byref(b)))
+        return b.value
+
     def work(self):
         f, b = ctypes.c_double(), ctypes.c_double()
         _lib.check(_lib.lib().amigo_ldl_work(self._h, ctypes.byref(f), ctypes.byref(b)))

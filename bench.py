@@ -46,6 +46,11 @@ def parse():
     ap.add_argument("--no-scaling", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the other-precision and serving measurements")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--timing-steps", type=int, default=1,
                     help="solves timed per kernel class with HIP events (roofline)")
     return ap.parse_args()
@@ -178,9 +183,88 @@ def main():
         "kernel_ms": dict({k: v[0] for k, v in times.items()}, assemble_big=asm_ms),
         "roofline": roof,
     }
+    if not args.no_forms:
+        result["forms"] = forms(args, prob, opt, outs, lat)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(prob, opt, nfact, nsolve, args.cpu_threads)
     print(json.dumps(result))
+
+
+def _inertia_of_last_solve(opt, before):
+    return [tuple(i) for i in opt.solver.inertia_history[before:]]
+
+
+def forms(args, prob, opt, outs, lat):
+    """The mixed-precision form on the same workload, and the served form on
+    seeded request mixes (each compared with the same requests solved one
+    at a time in FP64)."""
+    import numpy as np
+    import torch
+
+    from amigo_amd.optimizer import Optimizer
+    from amigo_amd.serve import SolveServer, request_mix, run_poisson
+
+    res = {}
+    # Precision: the other form of the single large solve.
+    other = "mixed" if args.precision == "fp64" else "fp64"
+    h0 = len(opt.solver.inertia_history)
+    opt.optimize({"record_objective": False})
+    ref_inertia = _inertia_of_last_solve(opt, h0)
+    x_ref = opt.get_solution()
+    o2 = Optimizer(prob, order=args.order, precision=other, scaling=not args.no_scaling)
+    o2.optimize({"record_objective": False})
+    torch.cuda.synchronize()
+    lat2 = []
+    for _ in range(args.steps):
+        h1 = len(o2.solver.inertia_history)
+        t = time.perf_counter()
+        out2 = o2.optimize({"record_objective": False})
+        torch.cuda.synchronize()
+        lat2.append(time.perf_counter() - t)
+    x2 = o2.get_solution()
+    res["precision"] = {
+        "form": other,
+        "value": len(lat2) / sum(lat2),
+        "p50_ms": 1e3 * float(np.percentile(lat2, 50)),
+        "p99_ms": 1e3 * float(np.percentile(lat2, 99)),
+        "ipm_iterations": out2["num_iterations"],
+        "final_nlp_error": out2["iterations"][-1]["nlp_error"],
+        "inertia_sequence_equal": _inertia_of_last_solve(o2, h1) == ref_inertia,
+        "max_abs_diff_vs_" + args.precision: float(np.max(np.abs(x2 - x_ref))),
+        "fp64_fallbacks": o2.solver.fp64_fallbacks,
+    }
+    print(f"[bench] {other}: {res['precision']['value']:.3f} solves/s", file=sys.stderr, flush=True)
+    del o2
+
+    # Serving: an open-loop Poisson stream of cart-pole and 2-D elliptic
+    # requests, several in flight (continuous batching), against the same
+    # requests solved one at a time.
+    specs = request_mix(args.serve_requests, seed=args.seed + 1)
+    with SolveServer(slots=1) as srv:
+        t = time.perf_counter()
+        seq = [f.result() for f in [srv.submit(sp) for sp in specs]]
+        seq_rate = len(seq) / (time.perf_counter() - t)
+    with SolveServer(slots=args.serve_slots) as srv:
+        st = run_poisson(srv, specs, rate=args.serve_rate, seed=args.seed + 2)
+    same = all(np.array_equal(a["x"], b["x"]) for a, b in zip(seq, st["results"]))
+    res["serving"] = {
+        "requests": len(specs),
+        "mix": "70% cart-pole N~U{32..1024}, 30% 2-D elliptic side~U{32..192}",
+        "arrival_rate_per_s": args.serve_rate,
+        "slots": args.serve_slots,
+        "value": st["solves_per_s"],
+        "unit": "solves/s",
+        "p50_ms": st["p50_ms"],
+        "p99_ms": st["p99_ms"],
+        "converged": st["converged"],
+        "sequential_fp64_solves_per_s": seq_rate,
+        "speedup_vs_sequential": st["solves_per_s"] / seq_rate,
+        "results_identical_to_sequential": bool(same),
+        "peak_inflight": st["peak_inflight"],
+    }
+    print(f"[bench] serving: {st['solves_per_s']:.2f} solves/s (sequential {seq_rate:.2f})",
+          file=sys.stderr, flush=True)
+    return res
 
 
 def cpu_baseline(prob, opt, nfact, nsolve, threads):

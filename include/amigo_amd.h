@@ -153,6 +153,13 @@ int amigo_ldl_solve(amigo_ldl* h, int nrhs, double* x_dev, int ldx,
 int amigo_ldl_work(const amigo_ldl* h, double* factor_flops,
                    double* solve_bytes);
 
+/* Predicted device bytes of the numeric phase for this symbolic plan (front
+ * pools at delay_growth x the static sizes, index pools, solve and panel
+ * workspaces, plan arrays), the counterpart of the memory estimate of
+ * SparseLDL's symbolic analysis (include/sparse_ldl.h:3162-3220).  Used to
+ * admit solves against the HBM budget. */
+int amigo_ldl_memory(const amigo_ldl* h, long long* bytes);
+
 #ifdef __cplusplus
 }
 #endif
