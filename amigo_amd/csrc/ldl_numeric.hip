@@ -1323,65 +1323,8 @@ __global__ __launch_bounds__(NTP) void panel_kernel(DevState S,
   }
 }
 
-// Trailing update F[k:, k:] -= F[k:, kstart:kstart+kw] W[k:, :kw]^T on the
-// lower triangle, 64x64 tiles (sparse_ldl.h:1172-1197).
-__global__ __launch_bounds__(NT) void trailing_kernel(DevState S,
-                                                      const int* __restrict__ bnodes,
-                                                      const double* __restrict__ Wbase,
-                                                      const int64_t* __restrict__ woff) {
-  const int node = bnodes[blockIdx.z];
-  const double* W = Wbase + woff[blockIdx.z];
-  __shared__ double As[TILE][NBW + 1];
-  __shared__ double Bs[TILE][NBW + 1];
-  __shared__ int bad;
-  if (threadIdx.x == 0) bad = atomicAdd(&S.flags[0], 0);
-  __syncthreads();
-  if (bad != 0) return;
-  const int* ps = S.pstate + PS * node;
-  const int m = S.m[node];
-  const int k = ps[0], kstart = ps[2], kw = ps[3];
-  const int ti = blockIdx.x, tj = blockIdx.y;
-  if (kw == 0 || ti < tj) return;
-  const int i0 = k + ti * TILE, j0 = k + tj * TILE;
-  if (i0 >= m || j0 >= m) return;
-  const double* F0 = S.rpool + S.foff[node];
-  double* F = S.rpool + S.foff[node];
-  const int ld = m, ldw = m;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < TILE * kw; e += NT) {
-    const int r = e % TILE, q = e / TILE;
-    const int i = i0 + r, j = j0 + r;
-    As[r][q] = i < m ? F0[i + (size_t)(kstart + q) * ld] : 0.0;
-    Bs[r][q] = j < m ? W[j + (size_t)q * ldw] : 0.0;
-  }
-  __syncthreads();
-  const int tx = tid % 16, ty = tid / 16;
-  double acc[4][4] = {};
-  for (int q = 0; q < kw; q++) {
-    double a[4], b[4];
-#pragma unroll
-    for (int r = 0; r < 4; r++) a[r] = As[tx + 16 * r][q];
-#pragma unroll
-    for (int s = 0; s < 4; s++) b[s] = Bs[ty + 16 * s][q];
-#pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-      for (int s = 0; s < 4; s++) acc[r][s] += a[r] * b[s];
-  }
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int i = i0 + tx + 16 * r;
-    if (i >= m) continue;
-#pragma unroll
-    for (int s = 0; s < 4; s++) {
-      const int j = j0 + ty + 16 * s;
-      if (j < m && i >= j) F[i + (size_t)j * ld] -= acc[r][s];
-    }
-  }
-}
-
-// Trailing update on MFMA: F[k:, k:] -= F[k:, kstart:kstart+kw] W[k:, :kw]^T
-// on the lower triangle, 64x64 tiles staged through LDS in precision T
+// Trailing update on MFMA (sparse_ldl.h:1172-1197):
+// F[k:, k:] -= F[k:, kstart:kstart+kw] W[k:, :kw]^T on the lower triangle, 64x64 tiles staged through LDS in precision T
 // (double: v_mfma_f64_16x16x4f64; float, the mixed-precision form:
 // v_mfma_f32_16x16x4f32 with FP32 accumulation, result subtracted from the
 // FP64 front).  Four waves per tile, each a 32x32 quadrant of 2x2 MFMA

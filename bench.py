@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the other-precision and serving measurements")
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="MiB of one device copy run first (counter calibration)")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -78,6 +80,17 @@ def main():
         out = opt.optimize({"record_objective": False})
         return out
 
+    if args.copy_calibration > 0:
+        # One plain device-to-device copy of known size, so that hardware
+        # counters (FETCH_SIZE / WRITE_SIZE) read in the same run can be
+        # scaled to bytes.
+        nb = int(args.copy_calibration * 2**20) // 8
+        src = torch.ones(nb, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy calibration: {8 * nb} bytes read and written", file=sys.stderr, flush=True)
+        del src, dst
     for w in range(args.warmup):
         t = time.perf_counter()
         out = one_solve()
@@ -127,6 +140,7 @@ def main():
     times["level"] = (times["level"][0] + asm_ms, times["level"][1])
     cls = max(times, key=lambda k: times[k][0])
     ms, launches = times[cls]
+    ms = max(ms, 1e-9)
     if cls == "level":
         work_per_factor = fl_small
         per = work_per_factor * nfact_t * args.timing_steps
