@@ -91,8 +91,8 @@ class LdlNumeric {
   int num_levels() const { return int(level_ptr_.size()) - 1; }
   int max_front() const { return max_front_; }
   bool factored() const { return factored_; }
-  // 0: none (the reference's behaviour); 1: symmetric equilibration of
-  // the shifted matrix before the factorization (inertia is unchanged).
+  // 0: none (the reference's behaviour); 1: symmetric (Ruiz) equilibration
+  // of the shifted matrix before the factorization (inertia is unchanged).
   void set_scaling(int mode) { scaling_ = mode != 0; }
   // HIP-event timing of every launch, accumulated per kernel class.
   void set_timing(bool on) { timing_ = on; }
@@ -170,6 +170,7 @@ class LdlNumeric {
   void tk_end(hipStream_t s, int cls);
   void tk_collect();
   double* scale_ = nullptr;
+  double* scale_tmp_ = nullptr;
   const int* d_rowp_ = nullptr;
   const int* d_cols_ = nullptr;
 };

@@ -1428,26 +1428,33 @@ __global__ void finalize_big_kernel(DevState S, const int* __restrict__ bnodes,
   atomicMax(&S.flags[6], S.m[node]);
 }
 
-// Symmetric equilibration: sc[i] = 1 / sqrt(max_j |a_ij|) over row i of the
-// shifted matrix (1 for an empty row).  One wave per row.
+// Symmetric equilibration by Ruiz iteration: each pass replaces
+// s_i <- s_i / sqrt(max_j |s_i a_ij s_j|) over row i of the shifted matrix
+// (Jacobi style: sin -> sout), so that every row and column maximum of the
+// scaled matrix tends to 1.  One wave per row.  (The reference factors the
+// unscaled matrix; the scaling leaves the inertia unchanged and the solves
+// unscale.)
 __global__ void equilibrate_kernel(int n, const int* __restrict__ rowp,
                                    const int* __restrict__ cols,
                                    const double* __restrict__ vals,
                                    const double* __restrict__ shift,
-                                   double* __restrict__ sc) {
+                                   const double* __restrict__ sin,
+                                   double* __restrict__ sout) {
   const int lane = threadIdx.x & 63;
   const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int nw = (gridDim.x * blockDim.x) >> 6;
   for (int r = w; r < n; r += nw) {
     double m = 0.0;
+    const double sr = sin ? sin[r] : 1.0;
     for (int p = rowp[r] + lane; p < rowp[r + 1]; p += 64) {
+      const int c = cols[p];
       double v = vals[p];
-      if (cols[p] == r && shift) v += shift[r];
-      m = fmax(m, fabs(v));
+      if (c == r && shift) v += shift[r];
+      m = fmax(m, fabs(v * sr * (sin ? sin[c] : 1.0)));
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-    if (lane == 0) sc[r] = m > 0.0 && m < INFINITY ? 1.0 / sqrt(m) : 1.0;
+    if (lane == 0) sout[r] = m > 0.0 && m < INFINITY ? sr / sqrt(m) : sr;
   }
 }
 
@@ -2002,6 +2009,7 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
     if (upload(allocs_, rp, &d_rowp_, err)) return -2;
     if (upload(allocs_, cl, &d_cols_, err)) return -2;
     if (dalloc(allocs_, std::max(n, 1), &scale_, err)) return -2;
+    if (dalloc(allocs_, std::max(n, 1), &scale_tmp_, err)) return -2;
   }
   P_.n = n;
   P_.K = K;
@@ -2106,8 +2114,16 @@ int LdlNumeric::run_factor(const double* vals, const double* shift,
   if (scaling_) {
     int nb = (n_ * 64 + NT - 1) / NT;
     nb = std::min(std::max(nb, 1), 8192);
-    hipLaunchKernelGGL(equilibrate_kernel, dim3(nb), dim3(NT), 0, stream, n_,
-                       d_rowp_, d_cols_, vals, shift, scale_);
+    // Five Ruiz passes, alternating between the two scale buffers, the
+    // last one writing scale_.
+    constexpr int kPasses = 5;
+    const double* sin = nullptr;
+    for (int it = 0; it < kPasses; it++) {
+      double* sout = ((kPasses - 1 - it) % 2 == 0) ? scale_ : scale_tmp_;
+      hipLaunchKernelGGL(equilibrate_kernel, dim3(nb), dim3(NT), 0, stream, n_,
+                         d_rowp_, d_cols_, vals, shift, sin, sout);
+      sin = sout;
+    }
     HIPCHK(hipGetLastError());
   }
   const int nlev = num_levels();

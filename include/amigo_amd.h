@@ -123,9 +123,9 @@ int amigo_ldl_pivot_counts(const amigo_ldl* h, int* num_pivots,
                            int* num_delayed);
 
 /* Symmetric equilibration before each factorization: mode 0 = none (the
- * reference's behaviour, default), 1 = scale row/column i by
- * 1/sqrt(max_j |a_ij|).  Inertia is unchanged; solves unscale.  No
- * reference counterpart. */
+ * reference's behaviour, default), 1 = Ruiz scaling (five passes of
+ * s_i <- s_i / sqrt(max_j |s_i a_ij s_j|)).  Inertia is unchanged; solves
+ * unscale.  No reference counterpart. */
 int amigo_ldl_set_scaling(amigo_ldl* h, int mode);
 
 /* HIP-event timing of every factor/solve launch (on = 1; 2 also resets
