@@ -99,6 +99,7 @@ _SIGS = {
     ),
     "amigo_ldl_work": (ctypes.c_int, [ctypes.c_void_p, _c_dbl_p, _c_dbl_p]),
     "amigo_ldl_memory": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]),
+    "amigo_ldl_set_graphs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "amigo_ldl_get_stats": (ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p]),
     "amigo_ldl_set_scaling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "amigo_ldl_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),

@@ -323,6 +323,12 @@ int amigo_ldl_work(const amigo_ldl* h, double* factor_flops,
   return AMIGO_OK;
 }
 
+int amigo_ldl_set_graphs(amigo_ldl* h, int on) {
+  if (!h) return fail(AMIGO_ERR_ARG, "null handle");
+  h->num.set_graphs(on != 0);
+  return AMIGO_OK;
+}
+
 int amigo_ldl_memory(const amigo_ldl* h, long long* bytes) {
   if (!h || !bytes) return fail(AMIGO_ERR_ARG, "null argument");
   // Device bytes the numeric phase allocates for this plan: front pools

@@ -96,6 +96,12 @@ class LdlNumeric {
   void set_scaling(int mode) { scaling_ = mode != 0; }
   // HIP-event timing of every launch, accumulated per kernel class.
   void set_timing(bool on) { timing_ = on; }
+  // Replay captured HIP graphs of the factorization / solve launch
+  // sequences (default on).
+  void set_graphs(bool on) {
+    graphs_ = on;
+    if (!on) drop_graphs();
+  }
   void get_timing(double* ms, long long* counts) const {
     for (int c = 0; c < TK_COUNT; c++) {
       ms[c] = tk_ms_[c];
@@ -117,6 +123,43 @@ class LdlNumeric {
   int run_factor(const double* vals, const double* shift, FactorParams prm,
                  bool cholesky, bool sync_mode, hipStream_t stream,
                  std::string& err);
+  int enqueue_factor(const double* vals, const double* shift, FactorParams prm,
+                     bool cholesky, bool sync_mode, hipStream_t stream,
+                     std::string& err);
+  int finish_factor(hipStream_t stream, std::string& err);
+  int enqueue_solve(int nrhs, double* x, int ldx, hipStream_t stream, std::string& err);
+  void capture_factor(const double* vals, const double* shift, FactorParams prm,
+                      bool cholesky, hipStream_t stream);
+  void capture_solve(int nrhs, double* x, int ldx, hipStream_t stream);
+  void drop_graphs();
+  bool capture_stream();
+  hipStream_t cap_stream_ = nullptr;
+
+  // HIP graphs: the factorization's launch sequence (keyed by its buffers,
+  // stream and parameters) and the solve sequences of recent right-hand
+  // side buffers.
+  struct SolveGraph {
+    double* x;
+    int nrhs, ldx;
+    hipStream_t stream;
+    int maxp;
+    hipGraphExec_t exec;
+    unsigned long long last_use;
+  };
+  static constexpr size_t kMaxSolveGraphs = 8;
+  bool graphs_ = true;
+  hipGraphExec_t fgraph_ = nullptr;
+  const double* fg_vals_ = nullptr;
+  const double* fg_shift_ = nullptr;
+  hipStream_t fg_stream_ = nullptr;
+  double fg_ustab_ = 0.0, fg_tol_ = 0.0;
+  bool fg_chol_ = false, fg_mixed_ = false;
+  std::vector<SolveGraph> sgraphs_;
+  std::vector<SolveGraph> sseen_;   // eager solves seen once (no graph yet)
+  unsigned long long sgraph_clock_ = 0;
+  const double* fseen_vals_ = nullptr;
+  const double* fseen_shift_ = nullptr;
+  hipStream_t fseen_stream_ = nullptr;
   int ensure_pools(unsigned long long rcap, unsigned long long icap,
                    std::string& err);
 

@@ -1800,6 +1800,8 @@ int dalloc(std::vector<void*>& allocs, size_t count, T** d, std::string& err) {
 }  // namespace
 
 LdlNumeric::~LdlNumeric() {
+  drop_graphs();
+  if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
   for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
   for (void* p : allocs_) (void)hipFree(p);
   if (S_.rpool) (void)hipFree(S_.rpool);
@@ -2077,6 +2079,7 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
 
 int LdlNumeric::ensure_pools(unsigned long long rcap, unsigned long long icap,
                              std::string& err) {
+  if (rcap > S_.rcap || icap > S_.icap) drop_graphs();
   if (rcap > S_.rcap) {
     if (S_.rpool) HIPCHK(hipFree(S_.rpool));
     S_.rpool = nullptr;
@@ -2108,6 +2111,16 @@ int LdlNumeric::ensure_pools(unsigned long long rcap, unsigned long long icap,
 int LdlNumeric::run_factor(const double* vals, const double* shift,
                            FactorParams prm, bool cholesky, bool sync_mode,
                            hipStream_t stream, std::string& err) {
+  const int rc = enqueue_factor(vals, shift, prm, cholesky, sync_mode, stream, err);
+  if (rc != 0) return rc;
+  return finish_factor(stream, err);
+}
+
+// Launch sequence of one factorization.  Without sync_mode it makes no
+// host synchronization, so it can be captured into a HIP graph.
+int LdlNumeric::enqueue_factor(const double* vals, const double* shift,
+                               FactorParams prm, bool cholesky, bool sync_mode,
+                               hipStream_t stream, std::string& err) {
   const bool mixed = prm.mixed;
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, stream, S_);
   HIPCHK(hipGetLastError());
@@ -2152,13 +2165,15 @@ int LdlNumeric::run_factor(const double* vals, const double* shift,
       // the fronts of this level that exceed the workgroup limit.
       HIPCHK(hipStreamSynchronize(stream));
       int flag = 0;
-      HIPCHK(hipMemcpy(&flag, S_.flags, sizeof(int), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpyAsync(&flag, S_.flags, sizeof(int), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
       if (flag != 0) return flag;
       std::vector<int>& hm = h_m_;
       std::vector<int> hD(K_);
       hm.resize(K_);
-      HIPCHK(hipMemcpy(hm.data(), S_.m, K_ * sizeof(int), hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(hD.data(), S_.D, K_ * sizeof(int), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpyAsync(hm.data(), S_.m, K_ * sizeof(int), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipMemcpyAsync(hD.data(), S_.D, K_ * sizeof(int), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
       std::vector<int> ln, lr, lns;
       std::vector<int64_t> wo;
       int64_t tot = 0;
@@ -2182,12 +2197,17 @@ int LdlNumeric::run_factor(const double* vals, const double* shift,
         W_ = nullptr;
         HIPCHK(hipMalloc(&W_, tot * sizeof(double)));
         W_cap_ = tot;
+        drop_graphs();
       }
-      HIPCHK(hipMemcpy(d_big_woff_sync_, wo.data(), nlaunch * sizeof(int64_t),
-                       hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(d_sync_nodes_, ln.data(), nlaunch * sizeof(int), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(d_sync_root_, lr.data(), nlaunch * sizeof(int), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(d_sync_ns_, lns.data(), nlaunch * sizeof(int), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpyAsync(d_big_woff_sync_, wo.data(), nlaunch * sizeof(int64_t),
+                            hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(d_sync_nodes_, ln.data(), nlaunch * sizeof(int),
+                            hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(d_sync_root_, lr.data(), nlaunch * sizeof(int),
+                            hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(d_sync_ns_, lns.data(), nlaunch * sizeof(int),
+                            hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));  // the host lists are loop locals
       bnodes = d_sync_nodes_;
       broot = d_sync_root_;
       bns = d_sync_ns_;
@@ -2241,10 +2261,16 @@ int LdlNumeric::run_factor(const double* vals, const double* shift,
   }
   hipLaunchKernelGGL(fpos_kernel, dim3(std::min(K_, 4096)), dim3(NT), 0, stream, P_, S_, K_);
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Wait for a factorization and read its status, inertia and statistics.
+int LdlNumeric::finish_factor(hipStream_t stream, std::string& err) {
   HIPCHK(hipStreamSynchronize(stream));
   if (timing_) tk_collect();
   int flags[8];
-  HIPCHK(hipMemcpy(flags, S_.flags, sizeof(flags), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(flags, S_.flags, sizeof(flags), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
   if (flags[0] != 0) return flags[0];
   if (flags[4] != 0) {
     err = "non-finite pivot (the matrix holds NaN or Inf)";
@@ -2264,6 +2290,22 @@ int LdlNumeric::factor(const double* vals, const double* shift, double ustab,
   FactorParams prm{ustab, pivot_tol, mixed};
   factored_ = false;
   bool sync_mode = false;
+  // Replay of the captured launch sequence (same buffers, stream and
+  // parameters); a replay that needs the level-synchronous mode or larger
+  // pools falls through to the eager path below.
+  const bool graph_ok = graphs_ && !timing_;
+  if (graph_ok && fgraph_ && fg_vals_ == vals && fg_shift_ == shift && fg_stream_ == stream &&
+      fg_ustab_ == ustab && fg_tol_ == pivot_tol && fg_chol_ == cholesky && fg_mixed_ == mixed) {
+    HIPCHK(hipGraphLaunch(fgraph_, stream));
+    const int rc = finish_factor(stream, err);
+    if (rc == 0) {
+      max_p_ = max_front_seen_;
+      factored_ = true;
+      return 0;
+    }
+    if (rc != -7 && rc != -4) return rc;
+    sync_mode = rc == -7;
+  }
   for (int attempt = 0; attempt < 8; attempt++) {
     int rc = run_factor(vals, shift, prm, cholesky, sync_mode, stream, err);
     if (rc == -7) {
@@ -2289,6 +2331,15 @@ int LdlNumeric::factor(const double* vals, const double* shift, double ustab,
     // solve kernels' LDS sizing.
     max_p_ = max_front_seen_;
     factored_ = true;
+    // Capture once the same buffers have been factored eagerly twice (a
+    // handle factored once does not pay for a capture).
+    if (graph_ok && !sync_mode) {
+      if (fseen_vals_ == vals && fseen_shift_ == shift && fseen_stream_ == stream)
+        capture_factor(vals, shift, prm, cholesky, stream);
+      fseen_vals_ = vals;
+      fseen_shift_ = shift;
+      fseen_stream_ = stream;
+    }
     return 0;
   }
   err = "front pools could not hold the delayed pivots";
@@ -2301,23 +2352,56 @@ int LdlNumeric::solve(int nrhs, double* x, int ldx, hipStream_t stream,
     err = "solve called before a successful factor";
     return -3;
   }
-  const size_t lds = size_t(max_p_ + 64 * 64) * sizeof(double);
-  if (lds > 160 * 1024) {
-    err = "front too large for the LDS solve kernels";
-    return -1;
-  }
-  const int nlev = num_levels();
   const size_t ubneed = (size_t)S_.icap * std::max(nrhs, 1);
   if (ubneed > ub_cap_) {
     if (ub_) HIPCHK(hipFree(ub_));
     ub_ = nullptr;
     ub_cap_ = 0;
+    drop_graphs();
     if (hipMalloc(&ub_, ubneed * sizeof(double)) != hipSuccess) {
       err = "solve workspace allocation failed";
       return -5;
     }
     ub_cap_ = ubneed;
   }
+  if (graphs_ && !timing_) {
+    for (SolveGraph& g : sgraphs_) {
+      if (g.x == x && g.nrhs == nrhs && g.ldx == ldx && g.stream == stream && g.maxp == max_p_) {
+        HIPCHK(hipGraphLaunch(g.exec, stream));
+        g.last_use = ++sgraph_clock_;
+        return 0;
+      }
+    }
+  }
+  const int rc = enqueue_solve(nrhs, x, ldx, stream, err);
+  if (rc != 0) return rc;
+  if (timing_) {
+    HIPCHK(hipStreamSynchronize(stream));
+    tk_collect();
+  } else if (graphs_) {
+    // Capture on the second eager solve of the same buffer.
+    bool seen = false;
+    for (const SolveGraph& g : sseen_)
+      if (g.x == x && g.nrhs == nrhs && g.ldx == ldx && g.stream == stream && g.maxp == max_p_)
+        seen = true;
+    if (seen) {
+      capture_solve(nrhs, x, ldx, stream);
+    } else {
+      if (sseen_.size() >= 2 * kMaxSolveGraphs) sseen_.erase(sseen_.begin());
+      sseen_.push_back(SolveGraph{x, nrhs, ldx, stream, max_p_, nullptr, 0});
+    }
+  }
+  return 0;
+}
+
+int LdlNumeric::enqueue_solve(int nrhs, double* x, int ldx, hipStream_t stream,
+                              std::string& err) {
+  const size_t lds = size_t(max_p_ + 64 * 64) * sizeof(double);
+  if (lds > 160 * 1024) {
+    err = "front too large for the LDS solve kernels";
+    return -1;
+  }
+  const int nlev = num_levels();
   const int sb = std::min(4096, std::max(1, (n_ * std::max(nrhs, 1) + NT - 1) / NT));
   if (scaling_)
     hipLaunchKernelGGL(scale_vec_kernel, dim3(sb), dim3(NT), 0, stream, n_, nrhs, ldx,
@@ -2340,11 +2424,86 @@ int LdlNumeric::solve(int nrhs, double* x, int ldx, hipStream_t stream,
     hipLaunchKernelGGL(scale_vec_kernel, dim3(sb), dim3(NT), 0, stream, n_, nrhs, ldx,
                        (const double*)scale_, x);
   HIPCHK(hipGetLastError());
-  if (timing_) {
-    HIPCHK(hipStreamSynchronize(stream));
-    tk_collect();
-  }
   return 0;
+}
+
+// ---- HIP graphs of the factorization and solve launch sequences ----------
+void LdlNumeric::drop_graphs() {
+  if (fgraph_) (void)hipGraphExecDestroy(fgraph_);
+  fgraph_ = nullptr;
+  for (SolveGraph& g : sgraphs_) (void)hipGraphExecDestroy(g.exec);
+  sgraphs_.clear();
+}
+
+// Record the factorization's launches (async plan) as a graph; on any
+// failure the eager path simply stays in use.
+// Captures run on a private non-blocking stream: recording executes
+// nothing, the graph is launched on the caller's stream, and a capturing
+// non-blocking stream never makes other threads' legacy-stream work depend
+// on it.
+bool LdlNumeric::capture_stream() {
+  if (cap_stream_) return true;
+  if (hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking) != hipSuccess) {
+    cap_stream_ = nullptr;
+    (void)hipGetLastError();
+    return false;
+  }
+  return true;
+}
+
+void LdlNumeric::capture_factor(const double* vals, const double* shift, FactorParams prm,
+                                bool cholesky, hipStream_t stream) {
+  if (!capture_stream()) return;
+  if (fgraph_) (void)hipGraphExecDestroy(fgraph_);
+  fgraph_ = nullptr;
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) return;
+  std::string err;
+  const int rc = enqueue_factor(vals, shift, prm, cholesky, false, cap_stream_, err);
+  if (hipStreamEndCapture(cap_stream_, &g) != hipSuccess || rc != 0 || g == nullptr) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return;
+  }
+  hipGraphExec_t ex = nullptr;
+  if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess) {
+    fgraph_ = ex;
+    fg_vals_ = vals;
+    fg_shift_ = shift;
+    fg_stream_ = stream;
+    fg_ustab_ = prm.ustab;
+    fg_tol_ = prm.pivot_tol;
+    fg_chol_ = cholesky;
+    fg_mixed_ = prm.mixed;
+  }
+  (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
+}
+
+void LdlNumeric::capture_solve(int nrhs, double* x, int ldx, hipStream_t stream) {
+  if (!capture_stream()) return;
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) return;
+  std::string err;
+  const int rc = enqueue_solve(nrhs, x, ldx, cap_stream_, err);
+  if (hipStreamEndCapture(cap_stream_, &g) != hipSuccess || rc != 0 || g == nullptr) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return;
+  }
+  hipGraphExec_t ex = nullptr;
+  if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess) {
+    if (sgraphs_.size() >= kMaxSolveGraphs) {
+      size_t lru = 0;
+      for (size_t i = 1; i < sgraphs_.size(); i++)
+        if (sgraphs_[i].last_use < sgraphs_[lru].last_use) lru = i;
+      (void)hipGraphExecDestroy(sgraphs_[lru].exec);
+      sgraphs_.erase(sgraphs_.begin() + lru);
+    }
+    sgraphs_.push_back(SolveGraph{x, nrhs, ldx, stream, max_p_, ex, ++sgraph_clock_});
+  }
+  (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
 }
 
 int LdlNumeric::pivot_counts(int* np, int* nd, std::string& err) const {

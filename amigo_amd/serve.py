@@ -135,6 +135,9 @@ class SolveServer:
                     prob = build_problem(spec, self.device)
                     opt = Optimizer(prob, order=_order_for(spec), precision=self.precision,
                                     scaling=spec.get("scaling", self.scaling))
+                    # No HIP-graph capture beside other slots: a stream capture
+                    # makes the other threads' legacy-stream copies fail.
+                    opt.solver.ldl.set_graphs(False)
                     need = opt.solver.ldl.memory_bytes() + 8 * (prob.nnz + 24 * prob.n)
                     self._admit(need)
                     t_admit = time.perf_counter()

@@ -262,6 +262,10 @@ class SparseLDL:
         names = ["level", "panel", "schur_update", "solve_fwd", "solve_bwd", "assemble_big"]
         return {k: (ms[i], cnt[i]) for i, k in enumerate(names)}, (fs.value, fb.value)
 
+    def set_graphs(self, on=True):
+        """Replay captured HIP graphs of the factor / solve launch sequences."""
+        _lib.check(_lib.lib().amigo_ldl_set_graphs(self._h, int(bool(on))))
+
     def memory_bytes(self):
         """Predicted device bytes of the numeric phase (pools, workspaces)."""
         b = ctypes.c_longlong()

@@ -153,6 +153,14 @@ int amigo_ldl_solve(amigo_ldl* h, int nrhs, double* x_dev, int ldx,
 int amigo_ldl_work(const amigo_ldl* h, double* factor_flops,
                    double* solve_bytes);
 
+/* Replay of HIP graphs (on = 1, the default; 0 = eager launches): once the
+ * same value / shift buffers have been factored twice on a stream, the
+ * factorization's launch sequence is captured and later factorizations
+ * replay it; likewise the solve sequence per right-hand-side buffer.  A
+ * replay that finds more delayed pivots than the captured plan holds falls
+ * back to eager launches.  No reference counterpart. */
+int amigo_ldl_set_graphs(amigo_ldl* h, int on);
+
 /* Predicted device bytes of the numeric phase for this symbolic plan (front
  * pools at delay_growth x the static sizes, index pools, solve and panel
  * workspaces, plan arrays), the counterpart of the memory estimate of
