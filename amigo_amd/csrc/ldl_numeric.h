@@ -186,7 +186,9 @@ class LdlNumeric {
   int* d_sync_root_ = nullptr;
   int* d_sync_ns_ = nullptr;
   int delayed_ = 0, max_front_seen_ = 0;
-  std::vector<int> level_lds_;      // LDS doubles per level kernel
+  std::vector<int> level_lds_;      // LDS doubles of a level's larger fronts
+  std::vector<int> level_split_;    // first larger front of each level
+  std::vector<int> level_lds_small_;  // LDS doubles of a level's small fronts
   int* d_level_nodes_ = nullptr;
   int* d_big_mask_ = nullptr;       // [K] 1 if front is factored blocked
   std::vector<int> h_ns_, h_cs_;

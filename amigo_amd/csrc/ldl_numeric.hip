@@ -29,6 +29,7 @@ namespace {
 constexpr int NT = 256;          // threads per workgroup (4 wave64)
 constexpr int NB = 64;           // panel width of the blocked path
 constexpr int LDS_MAX_M = 128;   // largest front held in LDS (128 KiB)
+constexpr int SMALL_LDS_M = 64;  // fronts launched with at most 64^2 LDS doubles
 constexpr int BIG_M = 128;       // static fronts above this use the blocked path
 constexpr int TILE = 64;         // Schur-update tile
 constexpr int DCAP_BIG = 64;     // delayed pivots a blocked front is sized for
@@ -1931,6 +1932,20 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
     std::vector<int> fill(level_ptr_.begin(), level_ptr_.end() - 1);
     for (int k = 0; k < K; k++) level_nodes_[fill[level[k]]++] = k;
   }
+  // Inside a level, fronts whose workgroup LDS fits SMALL_LDS_M^2 come
+  // first and are launched with that smaller LDS size (several workgroups
+  // per CU); the others follow in a second launch.
+  auto lds_side = [&](int k) { return std::min(ns[k] + cs[k] + 32, LDS_MAX_M); };
+  level_split_.assign(nlev, 0);
+  level_lds_small_.assign(nlev, 0);
+  for (int l = 0; l < nlev; l++) {
+    auto b = level_nodes_.begin() + level_ptr_[l], e = level_nodes_.begin() + level_ptr_[l + 1];
+    auto mid = std::stable_partition(b, e, [&](int k) { return lds_side(k) <= SMALL_LDS_M; });
+    level_split_[l] = int(mid - level_nodes_.begin());
+    int mx = 0;
+    for (auto it = b; it != mid; ++it) mx = std::max(mx, lds_side(*it));
+    level_lds_small_[l] = mx * mx;
+  }
   std::vector<int> big(K, 0);
   max_front_ = 0;
   double rneed = 0.0, ineed = 0.0;
@@ -1984,7 +1999,7 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
         if (mcap > NTP) big_wide_[l] = 1;
         big_tiles_[l] = std::max(big_tiles_[l], (mcap + TILE - 1) / TILE);
       }
-      mx = std::max(mx, std::min(ns[k] + cs[k] + 32, LDS_MAX_M));
+      if (t >= level_split_[l]) mx = std::max(mx, std::min(ns[k] + cs[k] + 32, LDS_MAX_M));
     }
     level_lds_[l] = mx * mx;
     big_ptr_[l + 1] = int(big_nodes_.size());
@@ -2141,15 +2156,20 @@ int LdlNumeric::enqueue_factor(const double* vals, const double* shift,
   }
   const int nlev = num_levels();
   for (int l = 0; l < nlev; l++) {
-    const int cnt = level_ptr_[l + 1] - level_ptr_[l];
-    const int lds = level_lds_[l];
+    const int split = level_split_[l];
+    const int parts[2][3] = {{level_ptr_[l], split - level_ptr_[l], level_lds_small_[l]},
+                             {split, level_ptr_[l + 1] - split, level_lds_[l]}};
     tk_begin(stream);
-    hipLaunchKernelGGL(level_kernel, dim3(cnt), dim3(NT),
-                       size_t(lds + 2 * LDS_MAX_M) * sizeof(double), stream, P_, S_, vals,
-                       shift, d_level_nodes_ + level_ptr_[l], d_big_mask_, lds,
-                       prm.ustab, prm.pivot_tol, cholesky ? 1 : 0,
-                       sync_mode ? -1 : DCAP_BIG,
-                       (const double*)(scaling_ ? scale_ : nullptr));
+    for (const auto& pt : parts) {
+      if (pt[1] == 0) continue;
+      const int lds = pt[2];
+      hipLaunchKernelGGL(level_kernel, dim3(pt[1]), dim3(NT),
+                         size_t(lds + 2 * LDS_MAX_M) * sizeof(double), stream, P_, S_, vals,
+                         shift, d_level_nodes_ + pt[0], d_big_mask_, lds,
+                         prm.ustab, prm.pivot_tol, cholesky ? 1 : 0,
+                         sync_mode ? -1 : DCAP_BIG,
+                         (const double*)(scaling_ ? scale_ : nullptr));
+    }
     tk_end(stream, TK_LEVEL);
     HIPCHK(hipGetLastError());
     const int b0 = big_ptr_[l], nb = big_ptr_[l + 1] - big_ptr_[l];

@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--batch-problems", type=int, default=48)
+    ap.add_argument("--batch-slots", type=int, default=16)
     ap.add_argument("--timing-steps", type=int, default=1,
                     help="solves timed per kernel class with HIP events (roofline)")
     return ap.parse_args()
@@ -278,6 +280,30 @@ def forms(args, prob, opt, outs, lat):
     }
     print(f"[bench] serving: {st['solves_per_s']:.2f} solves/s (sequential {seq_rate:.2f})",
           file=sys.stderr, flush=True)
+
+    # Batching: independent cart-pole swing-ups with N ~ U{16..512} and
+    # seeded masses and length, all queued at once, several in flight.
+    cp = request_mix(args.batch_problems, seed=args.seed + 3, cart_pole_frac=1.0, cart_n=(16, 512))
+    with SolveServer(slots=1) as srv:
+        t = time.perf_counter()
+        seq = [f.result() for f in [srv.submit(sp) for sp in cp]]
+        seq_rate = len(seq) / (time.perf_counter() - t)
+    with SolveServer(slots=args.batch_slots) as srv:
+        t = time.perf_counter()
+        bat = [f.result() for f in [srv.submit(sp) for sp in cp]]
+        bat_rate = len(bat) / (time.perf_counter() - t)
+    res["batching"] = {
+        "problems": len(cp),
+        "workload": "cart-pole swing-up, N~U{16..512}, T=2 s, mc~U(0.5,2), mp~U(0.1,0.5), l~U(0.3,1)",
+        "slots": args.batch_slots,
+        "value": bat_rate,
+        "unit": "solves/s",
+        "converged": sum(r["converged"] for r in bat),
+        "sequential_fp64_solves_per_s": seq_rate,
+        "speedup_vs_sequential": bat_rate / seq_rate,
+        "results_identical_to_sequential": bool(all(np.array_equal(a["x"], b["x"]) for a, b in zip(seq, bat))),
+    }
+    print(f"[bench] batching: {bat_rate:.2f} solves/s (sequential {seq_rate:.2f})", file=sys.stderr, flush=True)
     return res
 
 
