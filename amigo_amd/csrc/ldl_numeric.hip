@@ -1523,11 +1523,28 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Stage L[0:p, 0:p] (ld) into Ls (ld p); all threads, ends with a barrier.
-__device__ __forceinline__ void stage_pivot_block(const double* L, int ld, int p, double* Ls) {
-  for (int e = threadIdx.x; e < p * p; e += NT) {
-    const int jj = e / p, ii = e - jj * p;
-    Ls[e] = L[ii + (size_t)jj * ld];
+// Pivot blocks L[0:p, 0:p] (p <= 64) are staged in LDS as packed lower
+// triangles: column j starts at tri_off(j, p).
+constexpr int PIVOT_TRI = 64 * 65 / 2;
+__device__ __forceinline__ int tri_off(int j, int p) { return j * p - ((j * (j - 1)) >> 1); }
+
+// All threads; 16 loads per thread are in flight before any is stored.
+// Ends with a barrier.
+__device__ __forceinline__ void stage_pivot_block(const double* __restrict__ L, int ld, int p,
+                                                  double* Ls) {
+  const int tot = p * p;
+  for (int e0 = threadIdx.x; e0 < tot; e0 += 16 * NT) {
+    double a[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int e = e0 + u * NT, j = e / p, i = e - j * p;
+      a[u] = (e < tot && i >= j) ? L[i + (size_t)j * ld] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int e = e0 + u * NT, j = e / p, i = e - j * p;
+      if (e < tot && i >= j) Ls[tri_off(j, p) + i - j] = a[u];
+    }
   }
   __syncthreads();
 }
@@ -1540,13 +1557,15 @@ __device__ __forceinline__ double wave_lower_solve(double tl, int l, int p, cons
   role = 0;
   for (int j = 0; j < p;) {
     const bool two = __builtin_amdgcn_readlane(vl, j) < 0;
+    const int cj = tri_off(j, p) - j;  // Ls[cj + i] = L[i, j]
     if (!two) {
       const double tj = readlane_d(tl, j);
-      if (l > j && l < p) tl -= Ls[l + j * p] * tj;
+      if (l > j && l < p) tl -= Ls[cj + l] * tj;
       j += 1;
     } else {
       const double t1 = readlane_d(tl, j), t2 = readlane_d(tl, j + 1);
-      if (l > j + 1 && l < p) tl -= Ls[l + j * p] * t1 + Ls[l + (j + 1) * p] * t2;
+      const int cj1 = tri_off(j + 1, p) - (j + 1);
+      if (l > j + 1 && l < p) tl -= Ls[cj + l] * t1 + Ls[cj1 + l] * t2;
       if (l == j) role = 1;
       if (l == j + 1) role = 2;
       j += 2;
@@ -1561,13 +1580,15 @@ __device__ __forceinline__ double wave_d_solve(double tl, int l, int p, int role
   const double up = __shfl(tl, min(l + 1, 63), 64);
   const double dn = __shfl(tl, max(l - 1, 0), 64);
   if (l >= p) return tl;
-  if (role == 0) return tl / Ls[l + l * p];
+  if (role == 0) return tl / Ls[tri_off(l, p)];
   if (role == 1) {
-    const double d11 = Ls[l + l * p], d21 = Ls[l + 1 + l * p], d22 = Ls[l + 1 + (l + 1) * p];
+    const int c = tri_off(l, p);
+    const double d11 = Ls[c], d21 = Ls[c + 1], d22 = Ls[tri_off(l + 1, p)];
     const double inv = 1.0 / (d11 * d22 - d21 * d21);
     return inv * (d22 * tl - d21 * up);
   }
-  const double d11 = Ls[l - 1 + (l - 1) * p], d21 = Ls[l + (l - 1) * p], d22 = Ls[l + l * p];
+  const int c = tri_off(l - 1, p);
+  const double d11 = Ls[c], d21 = Ls[c + 1], d22 = Ls[tri_off(l, p)];
   const double inv = 1.0 / (d11 * d22 - d21 * d21);
   return inv * (d11 * tl - d21 * dn);
 }
@@ -1576,18 +1597,141 @@ __device__ __forceinline__ double wave_d_solve(double tl, int l, int p, int role
 __device__ __forceinline__ double wave_upper_solve(double tl, int l, int p, const int* vars,
                                                    const double* Ls) {
   const int vl = l < p ? vars[l] : 0;
+  const int cl = tri_off(l, p) - l;  // Ls[cl + i] = L[i, l]
   for (int i = p - 1; i >= 0;) {
     if (__builtin_amdgcn_readlane(vl, i) >= 0) {
       const double ti = readlane_d(tl, i);
-      if (l < i) tl -= Ls[i + l * p] * ti;
+      if (l < i) tl -= Ls[cl + i] * ti;
       i -= 1;
     } else {
       const double t2 = readlane_d(tl, i), t1 = readlane_d(tl, i - 1);
-      if (l < i - 1) tl -= Ls[i + l * p] * t2 + Ls[i - 1 + l * p] * t1;
+      if (l < i - 1) tl -= Ls[cl + i] * t2 + Ls[cl + i - 1] * t1;
       i -= 2;
     }
   }
   return tl;
+}
+
+// Front update of the forward solve: uo[i] = t[i] - sum_{j<p} L[i, j] t[j]
+// for the rows i in [p, m).  A thread takes one row and one of G column
+// groups (columns g, g + G, ...), with 16 loads in flight; the G partial sums
+// of a row are added in group order, so the result is deterministic.  red
+// holds NT doubles of LDS.  All threads; R = m - p is uniform.
+__device__ __forceinline__ void front_update(const double* __restrict__ L, int ld, int p, int m,
+                                             const double* t, double* __restrict__ uo,
+                                             double* red) {
+  const int tid = threadIdx.x, R = m - p;
+  if (R <= 0) return;
+  if (R < NT) {
+    const int G = max(1, min(NT / R, p));
+    double s = 0.0;
+    if (tid < G * R) {
+      const int g = tid / R, i = p + tid - g * R;
+      const double* Li = L + i;
+      for (int j0 = g; j0 < p; j0 += 16 * G) {
+        double a[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          const int j = j0 + u * G;
+          a[u] = j < p ? Li[(size_t)j * ld] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          const int j = j0 + u * G;
+          if (j < p) s += a[u] * t[j];
+        }
+      }
+    }
+    red[tid] = s;
+    __syncthreads();
+    if (tid < R) {
+      double a = t[p + tid];
+      for (int g = 0; g < G; g++) a -= red[g * R + tid];
+      uo[p + tid] = a;
+    }
+  } else {
+    for (int i = p + tid; i < m; i += NT) {
+      const double* Li = L + i;
+      double s = 0.0;
+      for (int j0 = 0; j0 < p; j0 += 16) {
+        double a[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) a[u] = j0 + u < p ? Li[(size_t)(j0 + u) * ld] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+          if (j0 + u < p) s += a[u] * t[j0 + u];
+      }
+      uo[i] = t[i] - s;
+    }
+  }
+}
+
+// Sum of v[u] over the 64 lanes for 16 values at once: four exchange steps
+// halve the values a lane holds (lane bits 5..2 select which one it keeps),
+// two more add over lane bits 1..0.  Lane l with (l & 3) == 0 returns the sum
+// of value ((l >> 2) & 15) read as bits 5..2 of l: index 8*b5 + 4*b4 + 2*b3 + b2.
+__device__ __forceinline__ double wave_reduce16(double (&v)[16], int lane) {
+#pragma unroll
+  for (int step = 0; step < 4; step++) {
+    const int half = 8 >> step, off = 32 >> step;
+    const bool hi = (lane & off) != 0;
+#pragma unroll
+    for (int u = 0; u < half; u++) {
+      double lo = v[u], up = v[u + half];
+      // Opaque copies: without them the two selects fold into a lane-varying
+      // index into v[], which is then lowered as a chain of selects over
+      // all 16 values.
+      asm volatile("" : "+v"(lo), "+v"(up));
+      const double keep = hi ? up : lo;
+      const double give = hi ? lo : up;
+      v[u] = keep + __shfl_xor(give, off, 64);
+    }
+  }
+  double s = v[0];
+  s += __shfl_xor(s, 2, 64);
+  s += __shfl_xor(s, 1, 64);
+  return s;
+}
+
+// Front update of the backward solve: t[j] -= sum_{i in [p,m)} L[i, j] t[i]
+// for j < p.  Wave w takes columns jb + w + NW u (u < 16) of each block of
+// 16 NW columns; its lanes run over the rows 64 at a time with 16 loads in
+// flight.  All threads; ends with a barrier.
+__device__ __forceinline__ void front_update_t(const double* __restrict__ L, int ld, int p,
+                                               int m, double* t) {
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63;
+  // The wave index is uniform: the column addresses stay in scalar registers.
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int jb = 0; jb < p; jb += 16 * NW) {
+    if (jb + w >= p) continue;  // no column of this block for this wave
+    double s[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) s[u] = 0.0;
+    // Columns past p read column p - 1 (in bounds) and are dropped below.
+    const double* Lc = L + (size_t)(jb + w) * ld;
+    const int ju = (p - 1 - jb - w) / NW;  // last u with a column below p
+    for (unsigned i = p + lane; i < (unsigned)m; i += 64) {
+      const double ti = t[i];
+      double a[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        // Uniform column base (scalar registers) plus the lane's row.
+        const double* cu = Lc + __builtin_amdgcn_readfirstlane(min(u, ju) * NW * ld);
+        a[u] = cu[i];
+      }
+#pragma unroll
+      for (int u = 0; u < 16; u++) s[u] += a[u] * ti;
+    }
+    const double tot = wave_reduce16(s, lane);
+    if ((lane & 3) == 0) {
+      const int u = 8 * ((lane >> 5) & 1) + 4 * ((lane >> 4) & 1) + 2 * ((lane >> 3) & 1) +
+                    ((lane >> 2) & 1);
+      const int j = jb + w + NW * u;
+      if (j < p) t[j] -= tot;
+    }
+  }
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -1598,13 +1742,18 @@ __device__ __forceinline__ double wave_upper_solve(double tl, int l, int p, cons
 // pivots follows, the front's own update vector u = t[p:m] - L21 t[0:p] is
 // stored for the parent, then t = D^-1 t goes to x.  Update vectors live in
 // ub at the front's index-pool offset (ld of one right-hand side = ubld).
+// Dynamic LDS: t (tcap >= m doubles), the packed pivot block, NT doubles.
 __global__ __launch_bounds__(NT) void solve_fwd_kernel(DevPlan P, DevState S,
                                                        const int* __restrict__ lnodes,
                                                        double* __restrict__ x,
                                                        int ldx, int nrhs,
                                                        double* __restrict__ ub,
-                                                       size_t ubld) {
-  extern __shared__ double t[];
+                                                       size_t ubld, int tcap) {
+  extern __shared__ double smem[];
+  double* t = smem;
+  double* Ls = smem + tcap;
+  double* red = Ls + PIVOT_TRI;
+  __shared__ int kmeta[4][NT];  // per child: delayed, update rows, offset, cmap start
   const int node = lnodes[blockIdx.x];
   const int p = S.npiv[node], m = S.m[node];
   const double* L = S.rpool + S.foff[node];
@@ -1613,30 +1762,53 @@ __global__ __launch_bounds__(NT) void solve_fwd_kernel(DevPlan P, DevState S,
   const int ns = P.ns[node], D = S.D[node];
   const int kb = P.kptr[node], ke = P.kptr[node + 1];
   const int* fpos = S.fpos + S.voff[node];
+  const bool wave = p <= 64;
+  if (wave) stage_pivot_block(L, ld, p, Ls);
   for (int r = 0; r < nrhs; r++) {
     double* xr = x + (size_t)r * ldx;
     double* ur = ub + (size_t)r * ubld;
     for (int j = tid; j < m; j += NT) {
-      const int v = vars[j];
-      t[j] = j < p ? xr[v >= 0 ? v : -v - 1] : 0.0;
+      double v = 0.0;
+      if (j < p) {
+        const int q = vars[j];
+        v = xr[q >= 0 ? q : -q - 1];
+      }
+      t[j] = v;
     }
     __syncthreads();
     int doff = ns;
-    for (int q = kb; q < ke; q++) {
-      const int c = P.kids[q];
-      const int pc = S.npiv[c], dc = S.ndel[c], wc = S.m[c] - pc;
-      const double* uc = ur + S.voff[c] + pc;
-      for (int ii = tid; ii < wc; ii += NT) {
-        int pos = child_row_pos(P, c, dc, doff, ns, D, ii);
-        if (pos < ns + D) pos = fpos[pos];
-        t[pos] += uc[ii];
+    for (int q0 = kb; q0 < ke; q0 += NT) {
+      const int nb = min(NT, ke - q0);
+      if (tid < nb) {
+        const int c = P.kids[q0 + tid];
+        const int pc = S.npiv[c];
+        kmeta[0][tid] = S.ndel[c];
+        kmeta[1][tid] = S.m[c] - pc;
+        kmeta[2][tid] = S.voff[c] + pc;
+        kmeta[3][tid] = P.cptr[c];
       }
-      doff += dc;
       __syncthreads();
+      for (int qq = 0; qq < nb; qq++) {
+        const int dc = kmeta[0][qq], wc = kmeta[1][qq];
+        const double* uc = ur + kmeta[2][qq];
+        const int* cm = P.cmap + kmeta[3][qq] - dc;  // cm[ii] for ii >= dc
+        for (int ii = tid; ii < wc; ii += NT) {
+          int pos;
+          if (ii < dc) {
+            pos = doff + ii;
+          } else {
+            const int s = cm[ii];
+            pos = s < ns ? s : s + D;
+          }
+          if (pos < ns + D) pos = fpos[pos];
+          t[pos] += uc[ii];
+        }
+        doff += kmeta[0][qq];
+        __syncthreads();
+      }
     }
-    if (p <= 64) {
-      double* Ls = t + m;
-      stage_pivot_block(L, ld, p, Ls);
+    double* uo = ur + S.voff[node];
+    if (wave) {
       double tl = 0.0;
       int role = 0;
       if (tid < 64) {
@@ -1644,12 +1816,7 @@ __global__ __launch_bounds__(NT) void solve_fwd_kernel(DevPlan P, DevState S,
         if (tid < p) t[tid] = tl;
       }
       __syncthreads();
-      double* uo = ur + S.voff[node];
-      for (int i = p + tid; i < m; i += NT) {
-        double s = t[i];
-        for (int j = 0; j < p; j++) s -= L[i + (size_t)j * ld] * t[j];
-        uo[i] = s;
-      }
+      front_update(L, ld, p, m, t, uo, red);
       if (tid < 64) {
         tl = wave_d_solve(tl, tid, p, role, Ls);
         if (tid < p) {
@@ -1673,13 +1840,8 @@ __global__ __launch_bounds__(NT) void solve_fwd_kernel(DevPlan P, DevState S,
       }
       __syncthreads();
     }
-    double* uo = ur + S.voff[node];
-    for (int i = p + tid; i < m; i += NT) {
-      double s = t[i];
-      for (int j = 0; j < p; j++) s -= L[i + (size_t)j * ld] * t[j];
-      uo[i] = s;
-    }
-    __syncthreads();  // t[0:p] is read above and rescaled below
+    front_update(L, ld, p, m, t, uo, red);
+    __syncthreads();  // t[0:p) is read above and rescaled below
     if (tid == 0) {
       for (int i = 0; i < p;) {
         if (vars[i] >= 0) {
@@ -1706,18 +1868,22 @@ __global__ __launch_bounds__(NT) void solve_fwd_kernel(DevPlan P, DevState S,
 }
 
 // Backward: t = x[piv]; t -= L21^T x[rows below]; L11^T solve; x[piv] = t.
+// Dynamic LDS: t (tcap >= m doubles) then the packed pivot block.
 __global__ __launch_bounds__(NT) void solve_bwd_kernel(DevState S,
                                                        const int* __restrict__ lnodes,
                                                        double* __restrict__ x,
-                                                       int ldx, int nrhs) {
-  extern __shared__ double t[];  // m doubles: pivots then the other rows
+                                                       int ldx, int nrhs, int tcap) {
+  extern __shared__ double smem[];
+  double* t = smem;  // pivots then the other rows
+  double* Ls = smem + tcap;
   const int node = lnodes[blockIdx.x];
   const int p = S.npiv[node], m = S.m[node];
   if (p == 0) return;
   const double* L = S.rpool + S.foff[node];
   const int* vars = S.ipool + S.voff[node];
   const int ld = m, tid = threadIdx.x;
-  const int lane = tid & 63, wid = tid >> 6;
+  const bool wave = p <= 64;
+  if (wave) stage_pivot_block(L, ld, p, Ls);
   for (int r = 0; r < nrhs; r++) {
     double* xr = x + (size_t)r * ldx;
     for (int j = tid; j < m; j += NT) {
@@ -1725,18 +1891,8 @@ __global__ __launch_bounds__(NT) void solve_bwd_kernel(DevState S,
       t[j] = xr[v >= 0 ? v : -v - 1];
     }
     __syncthreads();
-    // One wave per pivot column: t[j] -= sum_{i>=p} L[i, j] t[i].
-    for (int j = wid; j < p; j += NT / 64) {
-      double s = 0.0;
-      for (int i = p + lane; i < m; i += 64) s += L[i + (size_t)j * ld] * t[i];
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-      if (lane == 0) t[j] -= s;
-    }
-    __syncthreads();
-    if (p <= 64) {
-      double* Ls = t + m;
-      stage_pivot_block(L, ld, p, Ls);
+    front_update_t(L, ld, p, m, t);
+    if (wave) {
       if (tid < 64) {
         const double tl = wave_upper_solve(tid < p ? t[tid] : 0.0, tid, p, vars, Ls);
         if (tid < p) {
@@ -2416,8 +2572,9 @@ int LdlNumeric::solve(int nrhs, double* x, int ldx, hipStream_t stream,
 
 int LdlNumeric::enqueue_solve(int nrhs, double* x, int ldx, hipStream_t stream,
                               std::string& err) {
-  const size_t lds = size_t(max_p_ + 64 * 64) * sizeof(double);
-  if (lds > 160 * 1024) {
+  const int tcap = max_p_;
+  const size_t lds = size_t(tcap + PIVOT_TRI + NT) * sizeof(double);
+  if (lds > 150 * 1024) {
     err = "front too large for the LDS solve kernels";
     return -1;
   }
@@ -2430,14 +2587,15 @@ int LdlNumeric::enqueue_solve(int nrhs, double* x, int ldx, hipStream_t stream,
     const int cnt = level_ptr_[l + 1] - level_ptr_[l];
     tk_begin(stream);
     hipLaunchKernelGGL(solve_fwd_kernel, dim3(cnt), dim3(NT), lds, stream, P_, S_,
-                       d_level_nodes_ + level_ptr_[l], x, ldx, nrhs, ub_, (size_t)S_.icap);
+                       d_level_nodes_ + level_ptr_[l], x, ldx, nrhs, ub_, (size_t)S_.icap,
+                       tcap);
     tk_end(stream, TK_SOLVE_FWD);
   }
   for (int l = nlev - 1; l >= 0; l--) {
     const int cnt = level_ptr_[l + 1] - level_ptr_[l];
     tk_begin(stream);
     hipLaunchKernelGGL(solve_bwd_kernel, dim3(cnt), dim3(NT), lds, stream, S_,
-                       d_level_nodes_ + level_ptr_[l], x, ldx, nrhs);
+                       d_level_nodes_ + level_ptr_[l], x, ldx, nrhs, tcap);
     tk_end(stream, TK_SOLVE_BWD);
   }
   if (scaling_)
