@@ -30,6 +30,7 @@ constexpr int NT = 256;          // threads per workgroup (4 wave64)
 constexpr int NB = 64;           // panel width of the blocked path
 constexpr int LDS_MAX_M = 128;   // largest front held in LDS (128 KiB)
 constexpr int SMALL_LDS_M = 64;  // fronts launched with at most 64^2 LDS doubles
+constexpr int KID_BATCH = 64;    // children whose metadata a workgroup holds at once
 constexpr int BIG_M = 128;       // static fronts above this use the blocked path
 constexpr int TILE = 64;         // Schur-update tile
 constexpr int DCAP_BIG = 64;     // delayed pivots a blocked front is sized for
@@ -42,6 +43,13 @@ constexpr int NBR = 16;          // pivots per panel when the panel rows fit in 
 struct VI {
   double v;
   int i;
+};
+
+// A child front as its parent's workgroup reads it: pool offsets, pivots,
+// delayed pivots, order and the start of its contribution-row map.
+struct KidMeta {
+  int64_t foff, voff;
+  int pc, dc, mc, cptr;
 };
 
 __device__ __forceinline__ VI vi_best(VI a, VI b) {
@@ -359,7 +367,22 @@ __device__ __forceinline__ unsigned long long wave_max_key(unsigned long long x)
   const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), 63);
   return ((unsigned long long)hi << 32) | lo;
 }
+// Maximum over each row of 16 lanes, left in the row's last lane.
+__device__ __forceinline__ unsigned long long row_max_key(unsigned long long x) {
+  AMIGO_DPP_MAX_STEP(x, 0x111, 0xf, 0xf)  // row_shr:1
+  AMIGO_DPP_MAX_STEP(x, 0x112, 0xf, 0xf)  // row_shr:2
+  AMIGO_DPP_MAX_STEP(x, 0x113, 0xf, 0xf)  // row_shr:3
+  AMIGO_DPP_MAX_STEP(x, 0x114, 0xf, 0xe)  // row_shr:4
+  AMIGO_DPP_MAX_STEP(x, 0x118, 0xf, 0xc)  // row_shr:8
+  return x;
+}
 #undef AMIGO_DPP_MAX_STEP
+
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long x, int l) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), l);
+  return ((unsigned long long)hi << 32) | lo;
+}
 
 // Two block-wide argmax reductions (largest value, smallest index on ties;
 // lanes hold at most one candidate each, and a lane's index grows with the
@@ -379,14 +402,26 @@ __device__ void block_argmax2_dpp(VI& x, VI& y, VI* red) {
     red[NTH / 64 + wid] = VI{my ? __longlong_as_double((long long)(my - 1ull)) : -1.0, iy};
   }
   __syncthreads();
-  VI r = red[0], q = red[NTH / 64];
-#pragma unroll
-  for (int w = 1; w < NTH / 64; w++) {
-    r = vi_best(r, red[w]);
-    q = vi_best(q, red[NTH / 64 + w]);
-  }
-  x = r;
-  y = q;
+  // Across the waves, in every wave: the waves' x winners in lanes 0..NW-1
+  // (row 0), their y winners in lanes 32..32+NW-1 (row 2), one DPP row
+  // maximum each; equal keys go to the lowest lane, i.e. the lowest wave,
+  // whose rows come first.
+  constexpr int NW = NTH / 64;
+  static_assert(NW <= 16, "one wave winner per lane of a DPP row");
+  VI c = {-1.0, INT_MAX};
+  if (lane < NW)
+    c = red[lane];
+  else if (lane >= 32 && lane < 32 + NW)
+    c = red[NW + lane - 32];
+  const unsigned long long ck = cand_key(c);
+  const unsigned long long rk = row_max_key(ck);
+  const unsigned long long kx2 = readlane_u64(rk, 15), ky2 = readlane_u64(rk, 47);
+  const unsigned long long wx = __ballot(ck == kx2 && kx2 != 0ull) & 0xffffull;
+  const unsigned long long wy = __ballot(ck == ky2 && ky2 != 0ull) & (0xffffull << 32);
+  x.v = kx2 ? __longlong_as_double((long long)(kx2 - 1ull)) : -1.0;
+  x.i = wx ? __builtin_amdgcn_readlane(c.i, __ffsll((long long)wx) - 1) : INT_MAX;
+  y.v = ky2 ? __longlong_as_double((long long)(ky2 - 1ull)) : -1.0;
+  y.i = wy ? __builtin_amdgcn_readlane(c.i, __ffsll((long long)wy) - 1) : INT_MAX;
 }
 
 struct ColStats {
@@ -569,6 +604,8 @@ __global__ __launch_bounds__(NT) void level_kernel(
   __shared__ VI cred[2 * NT / 64];
   __shared__ int sh[4];
   __shared__ int cnt[4];
+  __shared__ KidMeta km[KID_BATCH];
+  __shared__ int rowpos[LDS_MAX_M];
   const int tid = threadIdx.x;
   const int node = lnodes[blockIdx.x];
   const int ns = P.ns[node], cs = P.cs[node];
@@ -576,11 +613,18 @@ __global__ __launch_bounds__(NT) void level_kernel(
 
   if (tid == 0) {
     sh[0] = -1;
+    sh[2] = 0;
     cnt[0] = cnt[1] = cnt[2] = cnt[3] = 0;
   }
+  __syncthreads();
+  // Delayed pivots of the children, summed in parallel (integer sum).
+  for (int q = kb + tid; q < ke; q += NT) {
+    const int nd = S.ndel[P.kids[q]];
+    if (nd) atomicAdd(&sh[2], nd);
+  }
+  __syncthreads();
   if (tid == 0 && atomicAdd(&S.flags[0], 0) == 0) {
-    int D = 0;
-    for (int q = kb; q < ke; q++) D += S.ndel[P.kids[q]];
+    int D = sh[2];
     const int m = ns + D + cs;
     if (dcap_big >= 0 && m > LDS_MAX_M && (!big_mask[node] || D > dcap_big)) {
       // The statically planned blocked launches cannot hold this front:
@@ -621,14 +665,33 @@ __global__ __launch_bounds__(NT) void level_kernel(
   // children (in child order), static contribution rows.
   for (int j = tid; j < ns; j += NT) vars[j] = P.nvars[P.nptr[node] + j];
   for (int j = tid; j < cs; j += NT) vars[ns + D + j] = P.crows[P.cptr[node] + j];
-  {
+  // Children's metadata, KID_BATCH children at a time (one batch for
+  // almost every front, kept for the extend-add below).
+  auto load_kids = [&](int q0, int nb) {
+    if (tid < nb) {
+      const int c = P.kids[q0 + tid];
+      km[tid].foff = S.foff[c];
+      km[tid].voff = S.voff[c];
+      km[tid].pc = S.npiv[c];
+      km[tid].dc = S.ndel[c];
+      km[tid].mc = S.m[c];
+      km[tid].cptr = P.cptr[c];
+    }
+    __syncthreads();
+  };
+  const bool one_batch = ke - kb <= KID_BATCH;
+  if (D > 0) {
     int off = ns;
-    for (int q = kb; q < ke; q++) {
-      const int c = P.kids[q];
-      const int nd = S.ndel[c];
-      const int* cv = S.ipool + S.voff[c] + S.npiv[c];
-      for (int t = tid; t < nd; t += NT) vars[off + t] = cv[t];
-      off += nd;
+    for (int q0 = kb; q0 < ke; q0 += KID_BATCH) {
+      const int nb = min(KID_BATCH, ke - q0);
+      load_kids(q0, nb);
+      for (int qq = 0; qq < nb; qq++) {
+        const int nd = km[qq].dc;
+        const int* cv = S.ipool + km[qq].voff + km[qq].pc;
+        for (int t = tid; t < nd; t += NT) vars[off + t] = cv[t];
+        off += nd;
+      }
+      __syncthreads();  // km is reloaded next
     }
   }
   if (big) {
@@ -667,28 +730,55 @@ __global__ __launch_bounds__(NT) void level_kernel(
     __syncthreads();
   }
 
-  // Extend-add of the children's contribution blocks, one child at a time.
+  // Extend-add of the children's contribution blocks, one child at a time
+  // (child order: deterministic sums).  The child's row positions in this
+  // front go to LDS first (a child's update rows all lie in this front, so
+  // there are at most m <= LDS_MAX_M of them); then each thread has 16 loads
+  // of the contribution block in flight.
   {
     int doff = ns;
-    for (int q = kb; q < ke; q++) {
-      const int c = P.kids[q];
-      const int pc = S.npiv[c], dc = S.ndel[c], mc = S.m[c];
-      const int w = mc - pc;
-      const double* Cf = S.rpool + S.foff[c];
-      const int tot = w * w;
-      for (int e = tid; e < tot; e += NT) {
-        const int ii = e % w, jj = e / w;
-        if (ii < jj) continue;
-        const int pi = child_row_pos(P, c, dc, doff, ns, D, ii);
-        const int pj = child_row_pos(P, c, dc, doff, ns, D, jj);
-        const double v = Cf[(pc + ii) + (size_t)(pc + jj) * mc];
-        if (pi >= pj)
-          F[pi + (size_t)pj * ld] += v;
-        else
-          F[pj + (size_t)pi * ld] += v;
+    for (int q0 = kb; q0 < ke; q0 += KID_BATCH) {
+      const int nb = min(KID_BATCH, ke - q0);
+      if (!one_batch || D == 0) load_kids(q0, nb);
+      for (int qq = 0; qq < nb; qq++) {
+        const int pc = km[qq].pc, dc = km[qq].dc, mc = km[qq].mc;
+        const int w = mc - pc;
+        const int* cm = P.cmap + km[qq].cptr - dc;  // cm[ii] for ii >= dc
+        for (int ii = tid; ii < w; ii += NT) {
+          int pos;
+          if (ii < dc) {
+            pos = doff + ii;
+          } else {
+            const int sidx = cm[ii];
+            pos = sidx < ns ? sidx : sidx + D;
+          }
+          rowpos[ii] = pos;
+        }
+        __syncthreads();
+        const double* Cf = S.rpool + km[qq].foff + pc + (size_t)pc * mc;
+        const int tot = w * w;
+        for (int e0 = tid; e0 < tot; e0 += 16 * NT) {
+          double a[16];
+#pragma unroll
+          for (int u = 0; u < 16; u++) {
+            const int e = e0 + u * NT, jj = e / w, ii = e - jj * w;
+            a[u] = (e < tot && ii >= jj) ? Cf[ii + (size_t)jj * mc] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < 16; u++) {
+            const int e = e0 + u * NT, jj = e / w, ii = e - jj * w;
+            if (e < tot && ii >= jj) {
+              const int pi = rowpos[ii], pj = rowpos[jj];
+              if (pi >= pj)
+                F[pi + (size_t)pj * ld] += a[u];
+              else
+                F[pj + (size_t)pi * ld] += a[u];
+            }
+          }
+        }
+        doff += dc;
+        __syncthreads();
       }
-      doff += dc;
-      __syncthreads();
     }
   }
 
@@ -855,15 +945,17 @@ __device__ __forceinline__ double ld_entry(const double* Ls, int t, int q, const
   return Ls[(q - 1) * NTP + t] * sh.de[q - 1] + l * sh.dd[q];
 }
 
+// s - sum_{q < kw} Ls[q][t] w[q], kw <= LS_COLS.  Fully unrolled with the
+// bound as a predicate, so all LDS reads are issued before the first FMA
+// waits on one.
 __device__ __forceinline__ double lds_dot(double s, const double* Ls, int t, const double* w,
                                           int kw) {
   double s1 = 0.0;
-  int q = 0;
-  for (; q + 2 <= kw; q += 2) {
-    s -= Ls[q * NTP + t] * w[q];
-    s1 -= Ls[(q + 1) * NTP + t] * w[q + 1];
+#pragma unroll
+  for (int q = 0; q < LS_COLS; q += 2) {
+    if (q < kw) s -= Ls[q * NTP + t] * w[q];
+    if (q + 1 < kw) s1 -= Ls[(q + 1) * NTP + t] * w[q + 1];
   }
-  if (q < kw) s -= Ls[q * NTP + t] * w[q];
   return s + s1;
 }
 
@@ -893,12 +985,37 @@ __device__ __forceinline__ void swap_lds_rows_values(double& s, double& s2, int 
 
 // Panel block <-> global front: columns [kstart, kstart + pw), rows >= the
 // column (lower triangle).
+// The batched form (loads in flight together) is for the panel's first
+// load; the reload after an interchange outside the panel, where registers
+// are scarce, goes column by column.
+template <bool BATCH>
 __device__ __forceinline__ void panel_load(const double* F, int ld, int m, int kstart, int pw,
                                            double* Ls) {
   const int t = threadIdx.x, rows = m - kstart;
-  if (t < rows)
-    for (int q = 0; q < pw; q++)
-      Ls[q * NTP + t] = t >= q ? F[kstart + t + (size_t)(kstart + q) * ld] : 0.0;
+  if (!BATCH) {
+    if (t < rows)
+      for (int q = 0; q < pw; q++)
+        Ls[q * NTP + t] = t >= q ? F[kstart + t + (size_t)(kstart + q) * ld] : 0.0;
+    return;
+  }
+  if (t < rows) {
+    // The row's loads are in flight PB at a time before their LDS stores.
+    constexpr int PB = 18;
+    static_assert(LS_COLS % PB == 0, "panel columns in whole batches");
+    const double* Fr = F + kstart + t + (size_t)kstart * ld;
+#pragma unroll
+    for (int q0 = 0; q0 < LS_COLS; q0 += PB) {
+      double a[PB];
+#pragma unroll
+      for (int u = 0; u < PB; u++) {
+        const int q = q0 + u;
+        a[u] = (q < pw && t >= q) ? Fr[(size_t)q * ld] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < PB; u++)
+        if (q0 + u < pw) Ls[(q0 + u) * NTP + t] = a[u];
+    }
+  }
 }
 __device__ __forceinline__ void panel_store(double* F, int ld, int m, int kstart, int pw,
                                             const double* Ls) {
@@ -946,7 +1063,7 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
   const int r = kstart + tid;
   const bool own = r < m;
   const int pw = min(LS_COLS, m - kstart);
-  panel_load(F, ld, m, kstart, pw, Ls);
+  panel_load<true>(F, ld, m, kstart, pw, Ls);
   __syncthreads();
   // Interchange of a < b, in LDS when both columns are in the panel,
   // otherwise through global memory (panel written back and reloaded).
@@ -957,7 +1074,7 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
       panel_store(F, ld, m, kstart, pw, Ls);
       __syncthreads();
       symm_swap_t<NTP>(F, ld, m, vars, a, b);
-      panel_load(F, ld, m, kstart, pw, Ls);
+      panel_load<false>(F, ld, m, kstart, pw, Ls);
     }
     swap_w_rows_t<NTP>(W, ldw, a, b, wcols);
     swap_lds_rows_values(s, s2, tid, a - kstart, b - kstart, sh);
@@ -1370,11 +1487,26 @@ __global__ __launch_bounds__(NT) void trailing_mfma_kernel(DevState S,
   const int ld = m, ldw = m;
   const int tid = threadIdx.x;
   const int kp = (kw + 3) & ~3;
-  for (int e = tid; e < TILE * kp; e += NT) {
-    const int r = e % TILE, q = e / TILE;
-    const int i = i0 + r, j = j0 + r;
-    As[r][q] = (i < m && q < kw) ? (T)F[i + (size_t)(kstart + q) * ld] : (T)0;
-    Bs[r][q] = (j < m && q < kw) ? (T)W[j + (size_t)q * ldw] : (T)0;
+  // Operand tiles into LDS, 2 x 8 loads per thread in flight.
+  const int tot = TILE * kp;
+  for (int e0 = tid; e0 < tot; e0 += 8 * NT) {
+    double av[8], bv[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int e = e0 + u * NT, r = e % TILE, q = e / TILE;
+      const int i = i0 + r, j = j0 + r;
+      const bool ok = e < tot && q < kw;
+      av[u] = (ok && i < m) ? F[i + (size_t)(kstart + q) * ld] : 0.0;
+      bv[u] = (ok && j < m) ? W[j + (size_t)q * ldw] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int e = e0 + u * NT, r = e % TILE, q = e / TILE;
+      if (e < tot) {
+        As[r][q] = (T)av[u];
+        Bs[r][q] = (T)bv[u];
+      }
+    }
   }
   __syncthreads();
   const int lane = tid & 63, wv = tid >> 6;
@@ -1396,6 +1528,9 @@ __global__ __launch_bounds__(NT) void trailing_mfma_kernel(DevState S,
 #pragma unroll
       for (int b = 0; b < 2; b++) acc[a][b] = mfma_16x16x4(av[a], bv[b], acc[a][b]);
   }
+  // Read all 16 old entries first (stores to F could alias later loads, so
+  // the compiler would otherwise wait on each load in turn).
+  double old[2][2][4];
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -1404,7 +1539,17 @@ __global__ __launch_bounds__(NT) void trailing_mfma_kernel(DevState S,
       for (int jj = 0; jj < 4; jj++) {
         const int i = i0 + wr + a * 16 + (sizeof(T) == 8 ? lk + 4 * jj : lk * 4 + jj);
         const int j = j0 + wc + b * 16 + lr;
-        if (i < m && j < m && i >= j) F[i + (size_t)j * ld] -= (double)acc[a][b][jj];
+        old[a][b][jj] = (i < m && j < m && i >= j) ? F[i + (size_t)j * ld] : 0.0;
+      }
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) {
+        const int i = i0 + wr + a * 16 + (sizeof(T) == 8 ? lk + 4 * jj : lk * 4 + jj);
+        const int j = j0 + wc + b * 16 + lr;
+        if (i < m && j < m && i >= j) F[i + (size_t)j * ld] = old[a][b][jj] - (double)acc[a][b][jj];
       }
 }
 
