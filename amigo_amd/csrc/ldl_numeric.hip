@@ -1659,8 +1659,8 @@ __global__ __launch_bounds__(NT) void fpos_kernel(DevPlan P, DevState S, int K) 
 // ---------------------------------------------------------------------------
 // Wave-synchronous pivot-block solves (p <= 64 pivots): lane l holds t[l],
 // the p x p pivot block L11 (with the D blocks on its diagonal) is staged in
-// LDS column-major, and each pivot step broadcasts one (two) value(s) with
-// readlane instead of a workgroup barrier.
+// LDS as a packed lower triangle, and each pivot step broadcasts one (two)
+// value(s) with readlane instead of a workgroup barrier.
 __device__ __forceinline__ double readlane_d(double v, int l) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)b, l);
