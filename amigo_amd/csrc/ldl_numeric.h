@@ -175,6 +175,7 @@ class LdlNumeric {
   std::vector<int> big_npanels_, big_tiles_;  // static launch plan per level
   std::vector<int> big_maxkids_;    // most children of a blocked front, per level
   std::vector<int> big_wide_;       // level has a front too tall for the register panel
+  std::vector<int> big_nth_;        // LDS-panel threads (= rows) of the level's launches
   std::vector<int> h_kptr_;         // children pointer (host copy)
   std::vector<int> h_m_;            // front sizes read back in sync mode
   const int* d_big_nodes_ = nullptr;

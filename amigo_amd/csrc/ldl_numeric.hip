@@ -913,10 +913,11 @@ __device__ __forceinline__ double panel_dot(double s, const double* __restrict__
   return (s + s1) + (s2 + s3);
 }
 
-// LDS-resident panel (fronts with at most NTP rows from the panel's first
-// pivot down): thread t owns row kstart + t, and the panel's columns
+// LDS-resident panel (fronts with at most NTH rows from the panel's first
+// pivot down, NTH = 256, 512 or 1024 threads by the level's largest planned
+// front): thread t owns row kstart + t, and the panel's columns
 // [kstart, kstart + NBR + 2) of all rows >= kstart live in LDS, column-major
-// (Ls[q * NTP + t]): not yet updated columns, overwritten by L as they are
+// (Ls[q * NTH + t]): not yet updated columns, overwritten by L as they are
 // factored.  A column update is conflict-free LDS reads against row k of
 // W = L D (formed from the L rows and the D blocks) broadcast through LDS;
 // the imax candidate column and symmetric interchanges inside the panel
@@ -938,23 +939,25 @@ struct PanelShared {
 };
 
 // Entry q of row t of W = L D (1x1 and 2x2 blocks of D).
+template <int NTH>
 __device__ __forceinline__ double ld_entry(const double* Ls, int t, int q, const PanelShared& sh) {
-  const double l = Ls[q * NTP + t];
+  const double l = Ls[q * NTH + t];
   if (sh.two[q] == 0) return l * sh.dd[q];
-  if (sh.two[q] == 1) return l * sh.dd[q] + Ls[(q + 1) * NTP + t] * sh.de[q];
-  return Ls[(q - 1) * NTP + t] * sh.de[q - 1] + l * sh.dd[q];
+  if (sh.two[q] == 1) return l * sh.dd[q] + Ls[(q + 1) * NTH + t] * sh.de[q];
+  return Ls[(q - 1) * NTH + t] * sh.de[q - 1] + l * sh.dd[q];
 }
 
 // s - sum_{q < kw} Ls[q][t] w[q], kw <= LS_COLS.  Fully unrolled with the
 // bound as a predicate, so all LDS reads are issued before the first FMA
 // waits on one.
+template <int NTH>
 __device__ __forceinline__ double lds_dot(double s, const double* Ls, int t, const double* w,
                                           int kw) {
   double s1 = 0.0;
 #pragma unroll
   for (int q = 0; q < LS_COLS; q += 2) {
-    if (q < kw) s -= Ls[q * NTP + t] * w[q];
-    if (q + 1 < kw) s1 -= Ls[(q + 1) * NTP + t] * w[q + 1];
+    if (q < kw) s -= Ls[q * NTH + t] * w[q];
+    if (q + 1 < kw) s1 -= Ls[(q + 1) * NTH + t] * w[q + 1];
   }
   return s + s1;
 }
@@ -988,14 +991,14 @@ __device__ __forceinline__ void swap_lds_rows_values(double& s, double& s2, int 
 // The batched form (loads in flight together) is for the panel's first
 // load; the reload after an interchange outside the panel, where registers
 // are scarce, goes column by column.
-template <bool BATCH>
+template <bool BATCH, int NTH>
 __device__ __forceinline__ void panel_load(const double* F, int ld, int m, int kstart, int pw,
                                            double* Ls) {
   const int t = threadIdx.x, rows = m - kstart;
   if (!BATCH) {
     if (t < rows)
       for (int q = 0; q < pw; q++)
-        Ls[q * NTP + t] = t >= q ? F[kstart + t + (size_t)(kstart + q) * ld] : 0.0;
+        Ls[q * NTH + t] = t >= q ? F[kstart + t + (size_t)(kstart + q) * ld] : 0.0;
     return;
   }
   if (t < rows) {
@@ -1013,48 +1016,51 @@ __device__ __forceinline__ void panel_load(const double* F, int ld, int m, int k
       }
 #pragma unroll
       for (int u = 0; u < PB; u++)
-        if (q0 + u < pw) Ls[(q0 + u) * NTP + t] = a[u];
+        if (q0 + u < pw) Ls[(q0 + u) * NTH + t] = a[u];
     }
   }
 }
+template <int NTH>
 __device__ __forceinline__ void panel_store(double* F, int ld, int m, int kstart, int pw,
                                             const double* Ls) {
   const int t = threadIdx.x, rows = m - kstart;
   if (t < rows)
     for (int q = 0; q < pw && q <= t; q++)
-      F[kstart + t + (size_t)(kstart + q) * ld] = Ls[q * NTP + t];
+      F[kstart + t + (size_t)(kstart + q) * ld] = Ls[q * NTH + t];
 }
 
 // Symmetric interchange a < b of the front when both columns lie in the
 // LDS panel (sparse_ldl.h:1252-1274 restricted to the stored lower
 // triangle): rows a, b of the earlier panels' columns are swapped in global
 // memory, everything else in LDS.  Ends with a barrier.
+template <int NTH>
 __device__ void panel_symm_swap(double* F, int ld, int m, int kstart, int* vars, int a, int b,
                                 double* Ls) {
   const int t0 = threadIdx.x;
   const int qa = a - kstart, qb = b - kstart;
   // Row part, columns c < a.
-  for (int c = t0; c < a; c += NTP) {
+  for (int c = t0; c < a; c += NTH) {
     if (c < kstart)
       swapd(F[a + (size_t)c * ld], F[b + (size_t)c * ld]);
     else
-      swapd(Ls[(c - kstart) * NTP + qa], Ls[(c - kstart) * NTP + qb]);
+      swapd(Ls[(c - kstart) * NTH + qa], Ls[(c - kstart) * NTH + qb]);
   }
   if (t0 == 0) {
-    swapd(Ls[qa * NTP + qa], Ls[qb * NTP + qb]);
+    swapd(Ls[qa * NTH + qa], Ls[qb * NTH + qb]);
     const int v = vars[a];
     vars[a] = vars[b];
     vars[b] = v;
   }
   // a < t < b: F[t, a] <-> F[b, t].
-  for (int t = a + 1 + t0; t < b; t += NTP)
-    swapd(Ls[qa * NTP + (t - kstart)], Ls[(t - kstart) * NTP + qb]);
+  for (int t = a + 1 + t0; t < b; t += NTH)
+    swapd(Ls[qa * NTH + (t - kstart)], Ls[(t - kstart) * NTH + qb]);
   // t > b: F[t, a] <-> F[t, b].
-  for (int t = b + 1 + t0; t < m; t += NTP)
-    swapd(Ls[qa * NTP + (t - kstart)], Ls[qb * NTP + (t - kstart)]);
+  for (int t = b + 1 + t0; t < m; t += NTH)
+    swapd(Ls[qa * NTH + (t - kstart)], Ls[qb * NTH + (t - kstart)]);
   __syncthreads();
 }
 
+template <int NTH>
 __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& nc, int& kw,
                           int kstart, bool bk, bool chol, double ustab, double pivot_tol,
                           double* Ls, PanelShared& sh) {
@@ -1063,30 +1069,30 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
   const int r = kstart + tid;
   const bool own = r < m;
   const int pw = min(LS_COLS, m - kstart);
-  panel_load<true>(F, ld, m, kstart, pw, Ls);
+  panel_load<true, NTH>(F, ld, m, kstart, pw, Ls);
   __syncthreads();
   // Interchange of a < b, in LDS when both columns are in the panel,
   // otherwise through global memory (panel written back and reloaded).
   auto interchange = [&](int a, int b, int wcols, double& s, double& s2) {
     if (b - kstart < pw) {
-      panel_symm_swap(F, ld, m, kstart, vars, a, b, Ls);
+      panel_symm_swap<NTH>(F, ld, m, kstart, vars, a, b, Ls);
     } else {
-      panel_store(F, ld, m, kstart, pw, Ls);
+      panel_store<NTH>(F, ld, m, kstart, pw, Ls);
       __syncthreads();
-      symm_swap_t<NTP>(F, ld, m, vars, a, b);
-      panel_load<false>(F, ld, m, kstart, pw, Ls);
+      symm_swap_t<NTH>(F, ld, m, vars, a, b);
+      panel_load<false, NTH>(F, ld, m, kstart, pw, Ls);
     }
-    swap_w_rows_t<NTP>(W, ldw, a, b, wcols);
+    swap_w_rows_t<NTH>(W, ldw, a, b, wcols);
     swap_lds_rows_values(s, s2, tid, a - kstart, b - kstart, sh);
   };
   while (kw < NBR && k < nc) {
     // Row k of W = L D, one entry per thread.
-    if (tid < kw) sh.wrow[tid] = ld_entry(Ls, k - kstart, tid, sh);
+    if (tid < kw) sh.wrow[tid] = ld_entry<NTH>(Ls, k - kstart, tid, sh);
     __syncthreads();
     double s = 0.0, s2 = 0.0;
     VI a = {-1.0, INT_MAX}, b = {-1.0, INT_MAX};
     if (own && r >= k) {
-      s = lds_dot(Ls[kw * NTP + tid], Ls, tid, sh.wrow, kw);
+      s = lds_dot<NTH>(Ls[kw * NTH + tid], Ls, tid, sh.wrow, kw);
       W[r + (size_t)kw * ldw] = s;
       if (r == k) sh.sval[0] = s;
       if (r > k) {
@@ -1102,7 +1108,7 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
     if (chol) {
       __syncthreads();
     } else {
-      block_argmax2_dpp<NTP>(a, b, sh.red);
+      block_argmax2_dpp<NTH>(a, b, sh.red);
       const double akk = fabs(sh.sval[0]);
       const bool has_cand = k + 1 < nc && a.i > k && a.i < nc;
       const int imax = has_cand ? a.i : 0;
@@ -1113,21 +1119,21 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
       if (ok1) {
         zero = bk && fmax(akk, colmax) == 0.0;
       } else if (has_cand) {
-        if (tid < kw) sh.wrow2[tid] = ld_entry(Ls, imax - kstart, tid, sh);
+        if (tid < kw) sh.wrow2[tid] = ld_entry<NTH>(Ls, imax - kstart, tid, sh);
         // Not yet updated column imax, rows >= k: rows below imax from its
         // column, rows k..imax-1 from row imax (the columns r < imax).
         double fi = 0.0;
         if (own && r >= k) {
           const int qi = imax - kstart;
           if (r >= imax)
-            fi = qi < pw ? Ls[qi * NTP + tid] : F[r + (size_t)imax * ld];
+            fi = qi < pw ? Ls[qi * NTH + tid] : F[r + (size_t)imax * ld];
           else
-            fi = tid < pw ? Ls[tid * NTP + qi] : F[imax + (size_t)r * ld];
+            fi = tid < pw ? Ls[tid * NTH + qi] : F[imax + (size_t)r * ld];
         }
         __syncthreads();
         VI rr = {-1.0, INT_MAX}, dummy = {-1.0, INT_MAX};
         if (own && r >= k) {
-          s2 = lds_dot(fi, Ls, tid, sh.wrow2, kw);
+          s2 = lds_dot<NTH>(fi, Ls, tid, sh.wrow2, kw);
           W[r + (size_t)(kw + 1) * ldw] = s2;
           if (r == imax) {
             sh.sval[1] = s2;
@@ -1136,7 +1142,7 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
             rr = VI{fabs(s2), r};
           }
         }
-        block_argmax2_dpp<NTP>(rr, dummy, sh.red);
+        block_argmax2_dpp<NTH>(rr, dummy, sh.red);
         const double rowmax = rr.v;
         const double aii = fabs(sh.sval[1]);
         if (bk) {
@@ -1195,7 +1201,7 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
           v = d;
         else
           v = zero ? 0.0 : s * (1.0 / d);
-        Ls[q * NTP + tid] = v;
+        Ls[q * NTH + tid] = v;
       }
       if (tid == 0) {
         sh.dd[q] = d;
@@ -1207,13 +1213,13 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
       const double detinv = 1.0 / (d11 * d22 - d21 * d21);
       const double i00 = d22 * detinv, i01 = -d21 * detinv, i11 = d11 * detinv;
       if (own && r >= k + 2) {
-        Ls[q * NTP + tid] = s * i00 + s2 * i01;
-        Ls[(q + 1) * NTP + tid] = s * i01 + s2 * i11;
+        Ls[q * NTH + tid] = s * i00 + s2 * i01;
+        Ls[(q + 1) * NTH + tid] = s * i01 + s2 * i11;
       }
       if (tid == 0) {
-        Ls[q * NTP + q] = d11;
-        Ls[q * NTP + q + 1] = d21;
-        Ls[(q + 1) * NTP + q + 1] = d22;
+        Ls[q * NTH + q] = d11;
+        Ls[q * NTH + q + 1] = d21;
+        Ls[(q + 1) * NTH + q + 1] = d22;
         vars[k] = -(vars[k] + 1);
         vars[k + 1] = -(vars[k + 1] + 1);
         sh.dd[q] = d11;
@@ -1228,7 +1234,7 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
     kw += step;
     __syncthreads();
   }
-  panel_store(F, ld, m, kstart, pw, Ls);
+  panel_store<NTH>(F, ld, m, kstart, pw, Ls);
 }
 
 // ---------------------------------------------------------------------------
@@ -1240,14 +1246,15 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
 // update W[k:, kw] = F[k:, k] - F[k:, kstart:k] W[k, :kw]^T reads row k of W
 // from LDS, and the candidate / contribution maxima come out of the same
 // pass with one fused reduction.
-template <bool REG>
-__global__ __launch_bounds__(NTP) void panel_kernel(DevState S,
-                                                    const int* __restrict__ bnodes,
-                                                    const int* __restrict__ broot,
-                                                    double* __restrict__ Wbase,
-                                                    const int64_t* __restrict__ woff,
-                                                    double ustab,
-                                                    double pivot_tol, int chol) {
+template <bool REG, int NTH>
+__global__ __launch_bounds__(REG ? NTH : NTP) void panel_kernel(DevState S,
+                                                                const int* __restrict__ bnodes,
+                                                                const int* __restrict__ broot,
+                                                                double* __restrict__ Wbase,
+                                                                const int64_t* __restrict__ woff,
+                                                                double ustab,
+                                                                double pivot_tol, int chol,
+                                                                int lds_rows) {
   const int node = bnodes[blockIdx.x];
   const int root = broot[blockIdx.x];
   double* W = Wbase + woff[blockIdx.x];
@@ -1287,10 +1294,10 @@ __global__ __launch_bounds__(NTP) void panel_kernel(DevState S,
   // Fronts whose panel rows fit one row per thread take the LDS path; the
   // two paths are separate instantiations so that each gets its own
   // register allocation, and each returns on the other's fronts.
-  if ((m - kstart <= NTP) != REG) return;
+  if ((m - kstart <= lds_rows) != REG) return;
   if constexpr (REG) {
     extern __shared__ double Ls[];
-    panel_lds(F, W, vars, m, k, nc, kw, kstart, bk, chol != 0, ustab, pivot_tol, Ls, psh);
+    panel_lds<NTH>(F, W, vars, m, k, nc, kw, kstart, bk, chol != 0, ustab, pivot_tol, Ls, psh);
   } else {
   while (kw < NB && k < nc) {
     double* wk = W + (size_t)kw * ldw;
@@ -2282,8 +2289,10 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   std::vector<int> big_root, big_ns;
   std::vector<int64_t> big_woff;
   int64_t wmax = 0;
+  big_nth_.assign(nlev, NTP);
   for (int l = 0; l < nlev; l++) {
     int mx = 0;
+    int big_mcap = 0;
     int64_t wtot = 0;
     for (int t = level_ptr_[l]; t < level_ptr_[l + 1]; t++) {
       int k = level_nodes_[t];
@@ -2298,11 +2307,13 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
         big_maxkids_[l] = std::max(big_maxkids_[l],
                                    plan.children_ptr[k + 1] - plan.children_ptr[k]);
         if (mcap > NTP) big_wide_[l] = 1;
+        big_mcap = std::max(big_mcap, mcap);
         big_tiles_[l] = std::max(big_tiles_[l], (mcap + TILE - 1) / TILE);
       }
       if (t >= level_split_[l]) mx = std::max(mx, std::min(ns[k] + cs[k] + 32, LDS_MAX_M));
     }
     level_lds_[l] = mx * mx;
+    big_nth_[l] = big_mcap <= NTP / 4 ? NTP / 4 : big_mcap <= NTP / 2 ? NTP / 2 : NTP;
     big_ptr_[l + 1] = int(big_nodes_.size());
     wmax = std::max(wmax, wtot);
   }
@@ -2376,9 +2387,15 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
       HIPCHK(hipFuncSetAttribute((const void*)level_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (LDS_MAX_M * LDS_MAX_M + 2 * LDS_MAX_M) * (int)sizeof(double)));
-      HIPCHK(hipFuncSetAttribute((const void*)panel_kernel<true>,
+      HIPCHK(hipFuncSetAttribute((const void*)panel_kernel<true, NTP>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)PANEL_LDS_BYTES));
+      HIPCHK(hipFuncSetAttribute((const void*)panel_kernel<true, NTP / 2>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)PANEL_LDS_BYTES / 2));
+      HIPCHK(hipFuncSetAttribute((const void*)panel_kernel<true, NTP / 4>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)PANEL_LDS_BYTES / 4));
       HIPCHK(hipFuncSetAttribute((const void*)solve_fwd_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  150 * 1024));
@@ -2559,13 +2576,27 @@ int LdlNumeric::enqueue_factor(const double* vals, const double* shift,
     }
     for (int q = 0; q < npanels; q++) {
       tk_begin(stream);
-      hipLaunchKernelGGL(panel_kernel<true>, dim3(nlaunch), dim3(NTP), PANEL_LDS_BYTES, stream, S_,
-                         bnodes, broot, W_, woff, prm.ustab, prm.pivot_tol,
-                         cholesky ? 1 : 0);
+      // One row per thread: the level's planned fronts (or, level-
+      // synchronously, any front up to NTP rows) in LDS; taller ones in
+      // global memory.
+      const int nth = sync_mode ? NTP : big_nth_[l];
+      const size_t plds = sizeof(double) * LS_COLS * nth;
+      if (nth == NTP / 4)
+        hipLaunchKernelGGL((panel_kernel<true, NTP / 4>), dim3(nlaunch), dim3(nth), plds,
+                           stream, S_, bnodes, broot, W_, woff, prm.ustab, prm.pivot_tol,
+                           cholesky ? 1 : 0, nth);
+      else if (nth == NTP / 2)
+        hipLaunchKernelGGL((panel_kernel<true, NTP / 2>), dim3(nlaunch), dim3(nth), plds,
+                           stream, S_, bnodes, broot, W_, woff, prm.ustab, prm.pivot_tol,
+                           cholesky ? 1 : 0, nth);
+      else
+        hipLaunchKernelGGL((panel_kernel<true, NTP>), dim3(nlaunch), dim3(nth), plds, stream,
+                           S_, bnodes, broot, W_, woff, prm.ustab, prm.pivot_tol,
+                           cholesky ? 1 : 0, nth);
       if (big_wide_[l] || sync_mode)
-        hipLaunchKernelGGL(panel_kernel<false>, dim3(nlaunch), dim3(NTP), 0, stream, S_,
+        hipLaunchKernelGGL((panel_kernel<false, NTP>), dim3(nlaunch), dim3(NTP), 0, stream, S_,
                            bnodes, broot, W_, woff, prm.ustab, prm.pivot_tol,
-                           cholesky ? 1 : 0);
+                           cholesky ? 1 : 0, nth);
       tk_end(stream, TK_PANEL);
       tk_begin(stream);
       if (mixed)
