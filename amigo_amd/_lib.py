@@ -80,6 +80,36 @@ _SIGS = {
             ctypes.POINTER(ctypes.c_void_p),
         ],
     ),
+    "amigo_ldl_create_perm": (
+        ctypes.c_int,
+        [
+            ctypes.c_int,
+            _c_int_p,
+            _c_int_p,
+            ctypes.c_int,
+            ctypes.c_double,
+            ctypes.c_double,
+            ctypes.c_double,
+            _c_int_p,
+            ctypes.POINTER(ctypes.c_void_p),
+        ],
+    ),
+    "amigo_ldl_create_union": (
+        ctypes.c_int,
+        [
+            ctypes.c_int,
+            _c_int_p,
+            _c_int_p,
+            ctypes.c_int,
+            ctypes.POINTER(ctypes.c_void_p),
+            ctypes.POINTER(ctypes.c_void_p),
+        ],
+    ),
+    "amigo_ldl_set_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_int_p]),
+    "amigo_ldl_get_batch_inertia": (
+        ctypes.c_int,
+        [ctypes.c_void_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p],
+    ),
     "amigo_ldl_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "amigo_ldl_sizes": (ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
     "amigo_ldl_get_symbolic": (
@@ -136,11 +166,32 @@ class IpmState(ctypes.Structure):
     ]
 
 
+class IpmBatch(ctypes.Structure):
+    """amigo_ipm_batch (include/amigo_ipm.h)."""
+
+    _fields_ = [
+        ("num_problems", ctypes.c_int),
+        ("pseg", ctypes.c_void_p),
+        ("cseg", ctypes.c_void_p),
+        ("xseg", ctypes.c_void_p),
+        ("nblk_total", ctypes.c_int),
+        ("blk_total", ctypes.c_void_p),
+        ("first_total", ctypes.c_void_p),
+        ("count_total", ctypes.c_void_p),
+        ("nblk_primal", ctypes.c_int),
+        ("blk_primal", ctypes.c_void_p),
+        ("first_primal", ctypes.c_void_p),
+        ("count_primal", ctypes.c_void_p),
+        ("active", ctypes.c_void_p),
+    ]
+
+
 _V = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
 _PP = ctypes.POINTER(IpmProblem)
 _PS = ctypes.POINTER(IpmState)
+_PB = ctypes.POINTER(IpmBatch)
 _SIGS.update(
     {
         "amigo_ipm_last_error": (ctypes.c_char_p, []),
@@ -170,6 +221,27 @@ _SIGS.update(
         "amigo_elements_eval": (_I, [_I, _I, _V, _V, _I, _V, _D, _V, _V, _V, _V]),
         "amigo_gather_sum": (_I, [_I, _V, _V, _V, _V, _I, _V]),
         "amigo_sum_partials": (_I, [_V, _I, _V, _c_dbl_p, _V]),
+        "amigo_elements_eval_scaled": (_I, [_I, _I, _V, _V, _I, _V, _D, _V, _V, _V, _V, _V]),
+        "amigo_sum_partials_batch": (_I, [_V, _I, _V, _V, _V, _c_dbl_p, _V]),
+        "amigo_ipm_grid_blocks": (_I, [_I]),
+        "amigo_ipm_batch_residual": (_I, [_PB, _PP, _PS, _V, _V, _V, _V, _V, _c_dbl_p, _V]),
+        "amigo_ipm_batch_diagonal": (_I, [_PB, _PP, _PS, _V, _V]),
+        "amigo_ipm_batch_bound_dual_step": (_I, [_PB, _PP, _PS, _V, _V, _V, _V, _V]),
+        "amigo_ipm_batch_max_step": (_I, [_PB, _PP, _PS, _V, _V, _V, _V, _V, _V, _c_dbl_p, _V]),
+        "amigo_ipm_batch_apply_step": (_I, [_PB, _PP, _PS, _V, _V, _V, _V, _V, _PS, _V]),
+        "amigo_ipm_batch_reduce": (_I, [_I, _PB, _PP, _PS, _V, _V, _V, _V, _V, _V, _c_dbl_p, _V]),
+        "amigo_ipm_batch_reset_bound_multipliers": (_I, [_PB, _PP, _PS, _V, _D, _V]),
+        "amigo_ipm_batch_dual_residual_vector": (_I, [_PB, _PP, _PS, _V, _V, _V]),
+        "amigo_ipm_batch_initialize": (_I, [_PB, _V, _V, _V, _D, _D, _D, _D, _V, _V, _PS, _I, _I, _V]),
+        "amigo_ipm_batch_floor_slacks": (_I, [_PB, _PP, _PS, _V, _V, _V, _V, _D, _V, _V, _c_dbl_p, _V]),
+        "amigo_ipm_batch_scale_rows": (_I, [_PB, _V, _V, _V, _I, _V]),
+        "amigo_ipm_batch_ir_init": (_I, [_PB, _PP, _PS, _V, _V, _V, _V, _V, _V, _V, _V, _c_dbl_p, _V]),
+        "amigo_ipm_batch_ir_residual": (
+            _I,
+            [_PB, _PP, _PS, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V, _c_dbl_p, _V],
+        ),
+        "amigo_ipm_batch_ir_update": (_I, [_PB, _PP, _PS, _V, _V, _V, _V, _V, _V, _V]),
+        "amigo_ipm_batch_copy_state": (_I, [_PB, _PS, _PS, _I, _V]),
     }
 )
 

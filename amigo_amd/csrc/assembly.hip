@@ -233,18 +233,23 @@ __device__ void eval_elem(int type, const double* z, const double* d, double alp
 }
 
 // One thread per element: gather inputs, evaluate, store g and H (and the
-// Lagrangian partial sum per block into lpart when non-null).
+// Lagrangian partial sum per block into lpart when non-null).  With
+// alpha_blk, block b scales the objective by alpha_blk[b] (a batch of
+// problems, each padded to whole blocks); an element whose first input is
+// AMIGO_ELEM_PAD is padding and contributes nothing.
 template <int NC>
 __global__ __launch_bounds__(NT) void element_kernel(int type, int E, const int* __restrict__ idx,
                                                      const double* __restrict__ data, int ndata,
                                                      const double* __restrict__ x, double alpha,
+                                                     const double* __restrict__ alpha_blk,
                                                      double* __restrict__ gbuf,
                                                      double* __restrict__ hbuf, int want_h,
                                                      double* __restrict__ lpart) {
   __shared__ double red[NT / 64];
   const int e = blockIdx.x * NT + threadIdx.x;
+  if (alpha_blk) alpha = alpha_blk[blockIdx.x];
   double Lval = 0.0;
-  if (e < E) {
+  if (e < E && idx[(size_t)e * NC] != AMIGO_ELEM_PAD) {
     double z[NC], g[NC], H[NC * NC];
 #pragma unroll
     for (int q = 0; q < NC; q++) {
@@ -287,6 +292,18 @@ __global__ void gather_sum_kernel(int n, const int* __restrict__ ptr,
   }
 }
 
+// Batch: problem k's sum runs over part[lidx[lptr[k] .. lptr[k+1]]] in list
+// order (its blocks of every element group, groups in order), the same
+// sequence sum_partials_kernel adds for the problem alone.
+__global__ void sum_partials_batch_kernel(const double* part, int K, const int* lptr,
+                                          const int* lidx, double* out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  double s = 0.0;
+  for (int p = lptr[k]; p < lptr[k + 1]; p++) s += part[lidx[p]];
+  out[k] = s;
+}
+
 __global__ void sum_partials_kernel(const double* part, int nb, double* out) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     double s = 0.0;
@@ -326,6 +343,13 @@ int amigo_elem_num_inputs(int type) {
 int amigo_elements_eval(int type, int E, const int* idx, const double* data, int ndata,
                         const double* x, double alpha, double* gbuf, double* hbuf,
                         double* lpart, void* stream) {
+  return amigo_elements_eval_scaled(type, E, idx, data, ndata, x, alpha, nullptr, gbuf, hbuf,
+                                    lpart, stream);
+}
+
+int amigo_elements_eval_scaled(int type, int E, const int* idx, const double* data, int ndata,
+                               const double* x, double alpha, const double* alpha_blk,
+                               double* gbuf, double* hbuf, double* lpart, void* stream) {
   if (E <= 0) return 0;
   const int nb = (E + NT - 1) / NT;
   hipStream_t s = (hipStream_t)stream;
@@ -333,23 +357,23 @@ int amigo_elements_eval(int type, int E, const int* idx, const double* data, int
   switch (type) {
     case AMIGO_ELEM_LINK:
       hipLaunchKernelGGL(element_kernel<2>, dim3(nb), dim3(NT), 0, s, type, E, idx, data, ndata, x,
-                         alpha, gbuf, hbuf, want_h, lpart);
+                         alpha, alpha_blk, gbuf, hbuf, want_h, lpart);
       break;
     case AMIGO_ELEM_DI_INTERVAL:
       hipLaunchKernelGGL(element_kernel<8>, dim3(nb), dim3(NT), 0, s, type, E, idx, data, ndata, x,
-                         alpha, gbuf, hbuf, want_h, lpart);
+                         alpha, alpha_blk, gbuf, hbuf, want_h, lpart);
       break;
     case AMIGO_ELEM_ELLIPTIC2D:
       hipLaunchKernelGGL(element_kernel<7>, dim3(nb), dim3(NT), 0, s, type, E, idx, data, ndata, x,
-                         alpha, gbuf, hbuf, want_h, lpart);
+                         alpha, alpha_blk, gbuf, hbuf, want_h, lpart);
       break;
     case AMIGO_ELEM_ELLIPTIC3D:
       hipLaunchKernelGGL(element_kernel<9>, dim3(nb), dim3(NT), 0, s, type, E, idx, data, ndata, x,
-                         alpha, gbuf, hbuf, want_h, lpart);
+                         alpha, alpha_blk, gbuf, hbuf, want_h, lpart);
       break;
     case AMIGO_ELEM_CARTPOLE:
       hipLaunchKernelGGL(element_kernel<14>, dim3(nb), dim3(NT), 0, s, type, E, idx, data, ndata, x,
-                         alpha, gbuf, hbuf, want_h, lpart);
+                         alpha, alpha_blk, gbuf, hbuf, want_h, lpart);
       break;
     default:
       g_asm_err = "unknown element type";
@@ -380,6 +404,23 @@ int amigo_sum_partials(const double* part, int nb, double* out_dev, double* out_
   if (e != hipSuccess) return set_err("sum_partials_kernel", e);
   if (out_host) {
     e = hipMemcpyAsync(out_host, out_dev, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return set_err("hipMemcpyAsync", e);
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return set_err("hipStreamSynchronize", e);
+  }
+  return 0;
+}
+
+int amigo_sum_partials_batch(const double* part, int K, const int* lptr, const int* lidx,
+                             double* out_dev, double* out_host, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (K <= 0) return 0;
+  hipLaunchKernelGGL(sum_partials_batch_kernel, dim3((K + 63) / 64), dim3(64), 0, s, part, K,
+                     lptr, lidx, out_dev);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err("sum_partials_batch_kernel", e);
+  if (out_host) {
+    e = hipMemcpyAsync(out_host, out_dev, sizeof(double) * K, hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return set_err("hipMemcpyAsync", e);
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return set_err("hipStreamSynchronize", e);

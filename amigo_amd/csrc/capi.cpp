@@ -115,10 +115,10 @@ int amigo_ldl_create(int n, const int* rowp, const int* cols, int solver_type,
                                   delay_growth, order, nullptr, out);
 }
 
-int amigo_ldl_create_grouped(int n, const int* rowp, const int* cols,
-                             int solver_type, double ustab, double pivot_tol,
-                             double delay_growth, int order,
-                             const int* var_group, amigo_ldl** out) {
+namespace {
+int create_handle(int n, const int* rowp, const int* cols, int solver_type,
+                  double ustab, double pivot_tol, double delay_growth, int order,
+                  const int* var_group, const int* user_perm, amigo_ldl** out) {
   if (!out) return fail(AMIGO_ERR_ARG, "amigo_ldl_create: null output");
   *out = nullptr;
   if (!check_csr(n, rowp, cols))
@@ -127,6 +127,15 @@ int amigo_ldl_create_grouped(int n, const int* rowp, const int* cols,
   if (order < 0 || order > 3) return fail(AMIGO_ERR_ARG, "unknown ordering");
   if (solver_type != AMIGO_SOLVER_LDL && solver_type != AMIGO_SOLVER_CHOLESKY)
     return fail(AMIGO_ERR_ARG, "unknown solver type");
+  if (user_perm) {
+    std::vector<char> seen(n, 0);
+    for (int i = 0; i < n; i++) {
+      const int v = user_perm[i];
+      if (v < 0 || v >= n || seen[v])
+        return fail(AMIGO_ERR_ARG, "amigo_ldl_create_perm: perm is not a permutation");
+      seen[v] = 1;
+    }
+  }
   std::unique_ptr<amigo_ldl> h;
   try {
     h.reset(new amigo_ldl);
@@ -152,7 +161,8 @@ int amigo_ldl_create_grouped(int n, const int* rowp, const int* cols,
         if (var_group[i] < 0 || var_group[i] >= n)
           return fail(AMIGO_ERR_ARG, "variable group id out of range");
     symbolic_analysis(order, n, rowp, cols, h->plan,
-                      order == AMIGO_ORDER_NESTED_DISSECTION ? var_group : nullptr);
+                      order == AMIGO_ORDER_NESTED_DISSECTION ? var_group : nullptr,
+                      user_perm);
   } catch (const std::exception& e) {
     return fail(AMIGO_ERR_NOMEM, std::string("symbolic analysis: ") + e.what());
   }
@@ -160,6 +170,74 @@ int amigo_ldl_create_grouped(int n, const int* rowp, const int* cols,
   // pattern for it.
   h->rowp.assign(rowp, rowp + n + 1);
   h->cols.assign(cols, cols + rowp[n]);
+  *out = h.release();
+  return AMIGO_OK;
+}
+}  // namespace
+
+int amigo_ldl_create_grouped(int n, const int* rowp, const int* cols,
+                             int solver_type, double ustab, double pivot_tol,
+                             double delay_growth, int order,
+                             const int* var_group, amigo_ldl** out) {
+  return create_handle(n, rowp, cols, solver_type, ustab, pivot_tol, delay_growth,
+                       order, var_group, nullptr, out);
+}
+
+int amigo_ldl_create_perm(int n, const int* rowp, const int* cols, int solver_type,
+                          double ustab, double pivot_tol, double delay_growth,
+                          const int* perm, amigo_ldl** out) {
+  if (!perm) return fail(AMIGO_ERR_ARG, "amigo_ldl_create_perm: null perm");
+  return create_handle(n, rowp, cols, solver_type, ustab, pivot_tol, delay_growth,
+                       AMIGO_ORDER_AMD, nullptr, perm, out);
+}
+
+int amigo_ldl_create_union(int n, const int* rowp, const int* cols, int num_blocks,
+                           const amigo_ldl* const* blocks, amigo_ldl** out) {
+  if (!out) return fail(AMIGO_ERR_ARG, "amigo_ldl_create_union: null output");
+  *out = nullptr;
+  if (num_blocks < 1 || !blocks)
+    return fail(AMIGO_ERR_ARG, "amigo_ldl_create_union: need at least one block");
+  if (!check_csr(n, rowp, cols))
+    return fail(AMIGO_ERR_ARG,
+                "amigo_ldl_create_union: CSR must have sorted, in-range column indices");
+  std::vector<const SymbolicPlan*> plans(num_blocks);
+  std::vector<int> voff(num_blocks);
+  int v = 0;
+  long long nz = 0;
+  for (int b = 0; b < num_blocks; b++) {
+    const amigo_ldl* B = blocks[b];
+    if (!B) return fail(AMIGO_ERR_ARG, "amigo_ldl_create_union: null block handle");
+    if (B->solver_type != blocks[0]->solver_type || B->ustab != blocks[0]->ustab ||
+        B->pivot_tol != blocks[0]->pivot_tol || B->delay_growth != blocks[0]->delay_growth)
+      return fail(AMIGO_ERR_ARG, "amigo_ldl_create_union: blocks differ in solver parameters");
+    // Row lengths of the union must be the blocks' own (block-diagonal).
+    for (int i = 0; i < B->n; i++)
+      if (v + i >= n || rowp[v + i + 1] - rowp[v + i] != B->rowp[i + 1] - B->rowp[i])
+        return fail(AMIGO_ERR_ARG, "amigo_ldl_create_union: pattern is not the blocks' diagonal union");
+    plans[b] = &B->plan;
+    voff[b] = v;
+    v += B->n;
+    nz += B->nnz;
+  }
+  if (v != n || nz != rowp[n])
+    return fail(AMIGO_ERR_ARG, "amigo_ldl_create_union: sizes do not add up");
+  std::unique_ptr<amigo_ldl> h;
+  try {
+    h.reset(new amigo_ldl);
+    const amigo_ldl* B0 = blocks[0];
+    h->n = n;
+    h->nnz = rowp[n];
+    h->solver_type = B0->solver_type;
+    h->ustab = B0->ustab;
+    h->pivot_tol = B0->pivot_tol;
+    h->delay_growth = B0->delay_growth;
+    h->order = B0->order;
+    concat_plans(plans, voff, h->plan);
+    h->rowp.assign(rowp, rowp + n + 1);
+    h->cols.assign(cols, cols + rowp[n]);
+  } catch (const std::exception& e) {
+    return fail(AMIGO_ERR_NOMEM, std::string("amigo_ldl_create_union: ") + e.what());
+  }
   *out = h.release();
   return AMIGO_OK;
 }
@@ -235,6 +313,40 @@ int amigo_ldl_factor_shifted(amigo_ldl* h, const double* values_dev,
 int amigo_ldl_factor(amigo_ldl* h, const double* values_dev, int precision,
                      void* stream) {
   return amigo_ldl_factor_shifted(h, values_dev, nullptr, precision, stream);
+}
+
+int amigo_ldl_set_batch(amigo_ldl* h, int num_blocks, const int* node_seg) {
+  if (!h) return fail(AMIGO_ERR_ARG, "null handle");
+  if (num_blocks < 1 || !node_seg)
+    return fail(AMIGO_ERR_ARG, "amigo_ldl_set_batch: need num_blocks >= 1 and node_seg");
+  if (node_seg[0] != 0 || node_seg[num_blocks] != h->plan.num_nodes)
+    return fail(AMIGO_ERR_ARG, "amigo_ldl_set_batch: node_seg must run from 0 to num_nodes");
+  for (int b = 0; b < num_blocks; b++)
+    if (node_seg[b + 1] < node_seg[b])
+      return fail(AMIGO_ERR_ARG, "amigo_ldl_set_batch: node_seg must be nondecreasing");
+  int rc = ensure_device(h);
+  if (rc) return rc;
+  std::string err;
+  rc = h->num.set_batch(num_blocks, node_seg, err);
+  if (rc) return fail(rc, "set_batch: " + err);
+  return AMIGO_OK;
+}
+
+int amigo_ldl_get_batch_inertia(const amigo_ldl* h, int* npos, int* nneg, int* nzero,
+                                int* nnan) {
+  if (!h) return fail(AMIGO_ERR_ARG, "null handle");
+  if (!h->num.factored()) return fail(AMIGO_ERR_STATE, "no factorization");
+  if (h->num.batch_blocks() == 0) return fail(AMIGO_ERR_STATE, "no batch segments set");
+  h->num.batch_inertia(npos, nneg, nzero, nnan);
+  if (h->solver_type == AMIGO_SOLVER_CHOLESKY) {
+    const int B = h->num.batch_blocks();
+    for (int b = 0; b < B; b++) {
+      if (npos) npos[b] = 0;
+      if (nneg) nneg[b] = 0;
+      if (nzero) nzero[b] = 0;
+    }
+  }
+  return AMIGO_OK;
 }
 
 int amigo_ldl_get_inertia(const amigo_ldl* h, int* npos, int* nneg,

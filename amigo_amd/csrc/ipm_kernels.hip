@@ -29,6 +29,82 @@ int set_err(const char* where, hipError_t e) {
 
 __device__ __forceinline__ bool finite_bound(double b) { return !isinf(b); }
 
+// ---- batches of independent problems ---------------------------------------
+// A batch lays K problems end to end: problem k owns primal entries
+// pseg[k]..pseg[k+1], constraint entries cseg[k]..cseg[k+1] and x-length
+// entries xseg[k]..xseg[k+1]; its primal / constraint index lists hold the
+// problem's own (local) variable numbers.  Block b of a batched launch
+// belongs to problem prob[b]; that problem's blocks are first[k] ..
+// first[k] + count[k] - 1, where count[k] is the grid the problem gets when
+// it runs alone.  Each block then runs exactly the loop it runs alone (same
+// elements per thread, same partial slot order), so every per-problem
+// result, reductions included, is bitwise the single-problem result.  All
+// pointers are null for a single problem.
+struct BT {
+  const int* prob;
+  const int* first;
+  const int* count;
+  const int* pseg;
+  const int* cseg;
+  const int* xseg;
+  const int* active;  // [K] nonzero = the problem takes part (null = all)
+};
+
+struct View {
+  int k, bx, gx, slot, xo, p0, c0, n;
+  bool skip;
+};
+
+__device__ __forceinline__ View view(const BT& B, amigo_ipm_problem& P, amigo_ipm_state& S) {
+  View v;
+  if (!B.prob) {
+    v.k = 0;
+    v.bx = blockIdx.x;
+    v.gx = gridDim.x;
+    v.slot = blockIdx.x;
+    v.xo = v.p0 = v.c0 = 0;
+    v.n = P.n_primal + P.n_constraints;
+    v.skip = false;
+    return v;
+  }
+  const int k = B.prob[blockIdx.x];
+  v.k = k;
+  v.bx = blockIdx.x - B.first[k];
+  v.gx = B.count[k];
+  v.slot = blockIdx.x;
+  v.skip = B.active && B.active[k] == 0;
+  v.p0 = B.pseg[k];
+  v.c0 = B.cseg[k];
+  v.xo = B.xseg[k];
+  v.n = B.xseg[k + 1] - v.xo;
+  P.n_primal = B.pseg[k + 1] - v.p0;
+  P.n_constraints = B.cseg[k + 1] - v.c0;
+  P.primal_indices += v.p0;
+  P.constraint_indices += v.c0;
+  P.lbx += v.p0;
+  P.ubx += v.p0;
+  P.lbh += v.c0;
+  S.xlam += v.xo;
+  S.zl += v.p0;
+  S.zu += v.p0;
+  S.sl += v.p0;
+  S.su += v.p0;
+  return v;
+}
+
+__device__ __forceinline__ void shift_state(amigo_ipm_state& T, const View& v) {
+  T.xlam += v.xo;
+  T.zl += v.p0;
+  T.zu += v.p0;
+  T.sl += v.p0;
+  T.su += v.p0;
+}
+
+// Scalar of the block's problem: sb[k] in a batch, s otherwise.
+__device__ __forceinline__ double pick(double s, const double* sb, const View& v) {
+  return sb ? sb[v.k] : s;
+}
+
 // ---- reduction helpers ------------------------------------------------------
 enum Op { SUM = 0, MAX = 1, MIN = 2 };
 
@@ -80,6 +156,32 @@ __global__ void finish_kernel(const double* ws, int nb, int o0, int o1, int o2,
     for (int q = 0; q < NV; q++) out[q] = v[q];
 }
 
+// Batched final pass: block k combines problem k's partials (rows
+// first[k] .. first[k]+count[k]-1 of ws) exactly as finish_kernel combines a
+// single problem's, and writes out[k*NV + q].
+template <int NV>
+__global__ void finish_batch_kernel(BT B, const double* ws, int o0, int o1, int o2, int o3,
+                                    double* out) {
+  __shared__ double red[(NT / 64) * 4];
+  const int k = blockIdx.x;
+  if (B.active && B.active[k] == 0) return;
+  const int ops[4] = {o0, o1, o2, o3};
+  const int nb = B.count[k];
+  ws += (size_t)B.first[k] * NV;
+  double v[NV];
+  int opv[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    opv[q] = ops[q];
+    v[q] = ident(ops[q]);
+  }
+  for (int b = threadIdx.x; b < nb; b += NT)
+    for (int q = 0; q < NV; q++) v[q] = combine(v[q], ws[b * NV + q], opv[q]);
+  block_reduce<NV>(v, opv, red);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NV; q++) out[(size_t)k * NV + q] = v[q];
+}
+
 int nblocks(int n) {
   int b = (n + NT - 1) / NT;
   if (b < 1) b = 1;
@@ -120,14 +222,20 @@ int finish_and_copy(int NV, const int* ops, double* ws, int nb, double* host,
 // for every primal and constraint row (res is zeroed for the others by the
 // caller's layout: every row is one or the other).  Partials:
 // [||res||^2, dual_sq, primal_sq].
-__global__ void residual_kernel(amigo_ipm_problem P, amigo_ipm_state S, double mu,
+__global__ void residual_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S, double mu,
+                                const double* __restrict__ mub,
                                 const double* __restrict__ grad,
                                 double* __restrict__ res, double* ws) {
+  const View V = view(B, P, S);
+  if (V.skip) return;
+  mu = pick(mu, mub, V);
+  grad += V.xo;
+  res += V.xo;
   __shared__ double red[(NT / 64) * 3];
   double v[3] = {0.0, 0.0, 0.0};
   const int ops[3] = {SUM, SUM, SUM};
   const int total = P.n_primal + P.n_constraints;
-  for (int t = blockIdx.x * NT + threadIdx.x; t < total; t += gridDim.x * NT) {
+  for (int t = V.bx * NT + threadIdx.x; t < total; t += V.gx * NT) {
     if (t < P.n_primal) {
       const int i = t;
       const int idx = P.primal_indices[i];
@@ -155,14 +263,17 @@ __global__ void residual_kernel(amigo_ipm_problem P, amigo_ipm_state S, double m
   }
   block_reduce<3>(v, ops, red);
   if (threadIdx.x == 0)
-    for (int q = 0; q < 3; q++) ws[blockIdx.x * 3 + q] = v[q];
+    for (int q = 0; q < 3; q++) ws[V.slot * 3 + q] = v[q];
 }
 
 // Sigma (interior_point_backend.h:277-288).
-__global__ void diagonal_kernel(amigo_ipm_problem P, amigo_ipm_state S,
+__global__ void diagonal_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S,
                                 double* __restrict__ diag) {
+  const View V = view(B, P, S);
+  if (V.skip) return;
+  diag += V.xo;
   const int total = P.n_primal + P.n_constraints;
-  for (int t = blockIdx.x * NT + threadIdx.x; t < total; t += gridDim.x * NT) {
+  for (int t = V.bx * NT + threadIdx.x; t < total; t += V.gx * NT) {
     if (t < P.n_primal) {
       double s = 0.0;
       if (finite_bound(P.lbx[t])) s += S.zl[t] / S.sl[t];
@@ -175,11 +286,18 @@ __global__ void diagonal_kernel(amigo_ipm_problem P, amigo_ipm_state S,
 }
 
 // Bound-dual back-substitution (interior_point_backend.h:302-321).
-__global__ void bound_dual_step_kernel(amigo_ipm_problem P, amigo_ipm_state S,
-                                       double mu, const double* __restrict__ px,
+__global__ void bound_dual_step_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S,
+                                       double mu, const double* __restrict__ mub,
+                                       const double* __restrict__ px,
                                        double* __restrict__ dzl,
                                        double* __restrict__ dzu) {
-  for (int i = blockIdx.x * NT + threadIdx.x; i < P.n_primal; i += gridDim.x * NT) {
+  const View V = view(B, P, S);
+  if (V.skip) return;
+  mu = pick(mu, mub, V);
+  px += V.xo;
+  dzl += V.p0;
+  dzu += V.p0;
+  for (int i = V.bx * NT + threadIdx.x; i < P.n_primal; i += V.gx * NT) {
     const double dx = px[P.primal_indices[i]];
     double a = 0.0, b = 0.0;
     if (finite_bound(P.lbx[i])) {
@@ -198,15 +316,22 @@ __global__ void bound_dual_step_kernel(amigo_ipm_problem P, amigo_ipm_state S,
 // Fraction to the boundary (interior_point_backend.h:328-369): minimum step
 // and the first index (loop order, lower bound before upper) attaining it;
 // only candidates below 1 count, as the reference starts from 1.
-__global__ void max_step_kernel(amigo_ipm_problem P, amigo_ipm_state S, double tau,
+__global__ void max_step_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S, double tau,
+                                const double* __restrict__ taub,
                                 const double* __restrict__ px,
                                 const double* __restrict__ dzl,
                                 const double* __restrict__ dzu, double* ws) {
+  const View V = view(B, P, S);
+  if (V.skip) return;
+  tau = pick(tau, taub, V);
+  px += V.xo;
+  dzl += V.p0;
+  dzu += V.p0;
   __shared__ double rv[2 * (NT / 64)];
   __shared__ long long rk[2 * (NT / 64)];
   double ax = 1.0, az = 1.0;
   long long kx = LLONG_MAX, kz = LLONG_MAX;
-  for (int i = blockIdx.x * NT + threadIdx.x; i < P.n_primal; i += gridDim.x * NT) {
+  for (int i = V.bx * NT + threadIdx.x; i < P.n_primal; i += V.gx * NT) {
     const double dx = px[P.primal_indices[i]];
     if (finite_bound(P.lbx[i])) {
       if (dx < 0.0) {
@@ -275,16 +400,26 @@ __global__ void max_step_kernel(amigo_ipm_problem P, amigo_ipm_state S, double t
         kz = rk[2 * w + 1];
       }
     }
-    ws[blockIdx.x * 4 + 0] = ax;
-    ws[blockIdx.x * 4 + 1] = (double)kx;
-    ws[blockIdx.x * 4 + 2] = az;
-    ws[blockIdx.x * 4 + 3] = (double)kz;
+    ws[V.slot * 4 + 0] = ax;
+    ws[V.slot * 4 + 1] = (double)kx;
+    ws[V.slot * 4 + 2] = az;
+    ws[V.slot * 4 + 3] = (double)kz;
   }
 }
 
-__global__ void max_step_finish(amigo_ipm_problem P, const double* ws, int nb,
+__global__ void max_step_finish(BT B, int K, amigo_ipm_problem P, const double* ws, int nb,
                                 double* out) {
-  if (threadIdx.x != 0) return;
+  if (B.prob) {
+    // One thread per problem: its partial rows and its own index list.
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K || (B.active && B.active[k] == 0)) return;
+    nb = B.count[k];
+    ws += (size_t)B.first[k] * 4;
+    P.primal_indices += B.pseg[k];
+    out += (size_t)k * 4;
+  } else if (threadIdx.x != 0) {
+    return;
+  }
   double ax = 1.0, az = 1.0;
   double kx = 9.0e18, kz = 9.0e18;
   for (int b = 0; b < nb; b++) {
@@ -306,12 +441,23 @@ __global__ void max_step_finish(amigo_ipm_problem P, const double* ws, int nb,
 }
 
 // Trial step (interior_point_backend.h:375-392).
-__global__ void apply_step_kernel(amigo_ipm_problem P, amigo_ipm_state S, double ax,
-                                  double az, const double* __restrict__ dxlam,
+__global__ void apply_step_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S, double ax,
+                                  double az, const double* __restrict__ axb,
+                                  const double* __restrict__ azb,
+                                  const double* __restrict__ dxlam,
                                   const double* __restrict__ dzl,
                                   const double* __restrict__ dzu, int n,
                                   amigo_ipm_state T) {
-  for (int t = blockIdx.x * NT + threadIdx.x; t < n; t += gridDim.x * NT) {
+  const View V = view(B, P, S);
+  if (V.skip) return;
+  ax = pick(ax, axb, V);
+  az = pick(az, azb, V);
+  if (B.prob) n = V.n;
+  dxlam += V.xo;
+  dzl += V.p0;
+  dzu += V.p0;
+  shift_state(T, V);
+  for (int t = V.bx * NT + threadIdx.x; t < n; t += V.gx * NT) {
     T.xlam[t] = S.xlam[t] + ax * dxlam[t];
     if (t < P.n_primal) {
       const int i = t;
@@ -348,10 +494,17 @@ enum RedKind {
   RK_ABS_SUMS = 9,     // [sum|zl|+|zu|, sum|y|, n_bounds, n_constr]
 };
 
-__global__ void reduce_kernel(int kind, amigo_ipm_problem P, amigo_ipm_state S,
-                              double mu, const double* __restrict__ a,
+__global__ void reduce_kernel(BT B, int kind, amigo_ipm_problem P, amigo_ipm_state S,
+                              double mu, const double* __restrict__ mub,
+                              const double* __restrict__ a,
                               const double* __restrict__ b,
                               const double* __restrict__ c, double* ws) {
+  const View V = view(B, P, S);
+  if (V.skip) return;
+  mu = pick(mu, mub, V);
+  if (a) a += V.xo;
+  if (b) b += V.xo;
+  if (c) c += V.xo;
   __shared__ double red[(NT / 64) * 4];
   int ops[4] = {SUM, SUM, SUM, SUM};
   if (kind == RK_COMPL) ops[2] = MIN;
@@ -362,7 +515,7 @@ __global__ void reduce_kernel(int kind, amigo_ipm_problem P, amigo_ipm_state S,
   if (kind == RK_KKT_ERR || kind == RK_MAX_COMP_DEV)
     for (int q = 0; q < 4; q++) v[q] = 0.0;  // the reference starts its maxima at 0
   const int total = P.n_primal + P.n_constraints;
-  for (int t = blockIdx.x * NT + threadIdx.x; t < total; t += gridDim.x * NT) {
+  for (int t = V.bx * NT + threadIdx.x; t < total; t += V.gx * NT) {
     const bool prim = t < P.n_primal;
     const int i = t, j = t - P.n_primal;
     const bool hl = prim && finite_bound(P.lbx[i]);
@@ -463,13 +616,17 @@ __global__ void reduce_kernel(int kind, amigo_ipm_problem P, amigo_ipm_state S,
   }
   block_reduce<4>(v, ops, red);
   if (threadIdx.x == 0)
-    for (int q = 0; q < 4; q++) ws[blockIdx.x * 4 + q] = v[q];
+    for (int q = 0; q < 4; q++) ws[V.slot * 4 + q] = v[q];
 }
 
 // z clamp into [mu/(kappa gap), kappa mu/gap] (backend.h:459-476).
-__global__ void reset_multipliers_kernel(amigo_ipm_problem P, amigo_ipm_state S,
-                                         double mu, double kappa) {
-  for (int i = blockIdx.x * NT + threadIdx.x; i < P.n_primal; i += gridDim.x * NT) {
+__global__ void reset_multipliers_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S,
+                                         double mu, const double* __restrict__ mub,
+                                         double kappa) {
+  const View V = view(B, P, S);
+  if (V.skip) return;
+  mu = pick(mu, mub, V);
+  for (int i = V.bx * NT + threadIdx.x; i < P.n_primal; i += V.gx * NT) {
     if (finite_bound(P.lbx[i])) {
       const double g = S.sl[i];
       S.zl[i] = fmax(fmin(S.zl[i], kappa * mu / g), mu / (kappa * g));
@@ -482,15 +639,23 @@ __global__ void reset_multipliers_kernel(amigo_ipm_problem P, amigo_ipm_state S,
 }
 
 // r_d = grad - zl + zu on primals, 0 elsewhere (backend.h:566-574).
-__global__ void dual_residual_kernel(amigo_ipm_problem P, amigo_ipm_state S,
+__global__ void dual_residual_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S,
                                      const double* __restrict__ grad,
                                      double* __restrict__ out, int n) {
-  for (int t = blockIdx.x * NT + threadIdx.x; t < n; t += gridDim.x * NT) out[t] = 0.0;
+  const View V = view(B, P, S);
+  if (V.skip) return;
+  if (B.prob) n = V.n;
+  out += V.xo;
+  for (int t = V.bx * NT + threadIdx.x; t < n; t += V.gx * NT) out[t] = 0.0;
 }
-__global__ void dual_residual_fill(amigo_ipm_problem P, amigo_ipm_state S,
+__global__ void dual_residual_fill(BT B, amigo_ipm_problem P, amigo_ipm_state S,
                                    const double* __restrict__ grad,
                                    double* __restrict__ out) {
-  for (int i = blockIdx.x * NT + threadIdx.x; i < P.n_primal; i += gridDim.x * NT) {
+  const View V = view(B, P, S);
+  if (V.skip) return;
+  grad += V.xo;
+  out += V.xo;
+  for (int i = V.bx * NT + threadIdx.x; i < P.n_primal; i += V.gx * NT) {
     const int idx = P.primal_indices[i];
     out[idx] = grad[idx] - S.zl[i] + S.zu[i];
   }
@@ -498,13 +663,22 @@ __global__ void dual_residual_fill(amigo_ipm_problem P, amigo_ipm_state S,
 
 // Starting point (backend.h:79-200): relaxed bounds, projection of the
 // primals into the interior, unit bound duals, slacks from the primals.
-__global__ void init_kernel(const double* __restrict__ lbx0,
+__global__ void init_kernel(BT B, const double* __restrict__ lbx0,
                             const double* __restrict__ ubx0, int np,
                             const int* __restrict__ pidx, double relax,
                             double cvt, double kappa1, double kappa2,
                             double* __restrict__ lbx, double* __restrict__ ubx,
                             amigo_ipm_state S, int do_relax, int do_project) {
-  for (int i = blockIdx.x * NT + threadIdx.x; i < np; i += gridDim.x * NT) {
+  amigo_ipm_problem P{np, 0, pidx, nullptr, lbx0, ubx0, nullptr};
+  const View V = view(B, P, S);
+  if (V.skip) return;
+  np = P.n_primal;
+  lbx0 += V.p0;
+  ubx0 += V.p0;
+  pidx += V.p0;
+  lbx += V.p0;
+  ubx += V.p0;
+  for (int i = V.bx * NT + threadIdx.x; i < np; i += V.gx * NT) {
     double lb = lbx0[i], ub = ubx0[i];
     if (do_relax && relax > 0) {
       if (finite_bound(lb)) lb = lb - fmin(cvt, relax * fmax(1.0, fabs(lb)));
@@ -541,14 +715,22 @@ __global__ void init_kernel(const double* __restrict__ lbx0,
 // Slack flooring (amigo/algorithm/bound_safeguards.py:16-87): where a stored
 // slack fell below s_min, reset it to min(max(mu/z, s_min), cap) and move the
 // primal to stay consistent.  ws[0] receives the count (per block).
-__global__ void floor_slacks_kernel(amigo_ipm_problem P, amigo_ipm_state S,
+__global__ void floor_slacks_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S,
                                     const double* __restrict__ lbx0,
                                     const double* __restrict__ ubx0, double mu,
-                                    double s_min, double slack_move, double* ws) {
+                                    const double* __restrict__ mub, double s_min,
+                                    const double* __restrict__ sminb, double slack_move,
+                                    double* ws) {
+  const View V = view(B, P, S);
+  if (V.skip) return;
+  mu = pick(mu, mub, V);
+  s_min = pick(s_min, sminb, V);
+  lbx0 += V.p0;
+  ubx0 += V.p0;
   __shared__ double red[(NT / 64) * 1];
   double v[1] = {0.0};
   const int ops[1] = {SUM};
-  for (int i = blockIdx.x * NT + threadIdx.x; i < P.n_primal; i += gridDim.x * NT) {
+  for (int i = V.bx * NT + threadIdx.x; i < P.n_primal; i += V.gx * NT) {
     const int idx = P.primal_indices[i];
     if (finite_bound(lbx0[i]) && S.sl[i] < s_min) {
       const double g = fmax(S.sl[i], 0.0);
@@ -570,7 +752,7 @@ __global__ void floor_slacks_kernel(amigo_ipm_problem P, amigo_ipm_state S,
     }
   }
   block_reduce<1>(v, ops, red);
-  if (threadIdx.x == 0) ws[blockIdx.x] = v[0];
+  if (threadIdx.x == 0) ws[V.slot] = v[0];
 }
 
 // ---- sparse matrix helpers -------------------------------------------------
@@ -630,8 +812,9 @@ __global__ void constraint_scale_kernel(int nc, const int* __restrict__ cidx,
 // Iterative refinement on the full 8-block system (linear_solver.py:109-264):
 // residual blocks, their inf-norms, and the condensed correction RHS.
 // ws partial layout: [max|e|, max|(px,dzl,dzu)|].
-__global__ void ir_residual_kernel(amigo_ipm_problem P, amigo_ipm_state S,
-                                   double mu, const double* __restrict__ grad,
+__global__ void ir_residual_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S,
+                                   double mu, const double* __restrict__ mub,
+                                   const double* __restrict__ grad,
                                    const double* __restrict__ rhs_saved,
                                    const double* __restrict__ Kpx,
                                    const double* __restrict__ px,
@@ -640,11 +823,23 @@ __global__ void ir_residual_kernel(amigo_ipm_problem P, amigo_ipm_state S,
                                    double* __restrict__ econd,
                                    double* __restrict__ ezl,
                                    double* __restrict__ ezu, double* ws) {
+  const View V = view(B, P, S);
+  if (V.skip) return;
+  mu = pick(mu, mub, V);
+  grad += V.xo;
+  rhs_saved += V.xo;
+  Kpx += V.xo;
+  px += V.xo;
+  econd += V.xo;
+  dzl += V.p0;
+  dzu += V.p0;
+  ezl += V.p0;
+  ezu += V.p0;
   __shared__ double red[(NT / 64) * 2];
   double v[2] = {0.0, 0.0};
   const int ops[2] = {MAX, MAX};
   const int total = P.n_primal + P.n_constraints;
-  for (int t = blockIdx.x * NT + threadIdx.x; t < total; t += gridDim.x * NT) {
+  for (int t = V.bx * NT + threadIdx.x; t < total; t += V.gx * NT) {
     if (t < P.n_primal) {
       const int i = t, idx = P.primal_indices[i];
       const bool fl = finite_bound(P.lbx[i]), fu = finite_bound(P.ubx[i]);
@@ -671,20 +866,26 @@ __global__ void ir_residual_kernel(amigo_ipm_problem P, amigo_ipm_state S,
   }
   block_reduce<2>(v, ops, red);
   if (threadIdx.x == 0) {
-    ws[blockIdx.x * 2 + 0] = v[0];
-    ws[blockIdx.x * 2 + 1] = v[1];
+    ws[V.slot * 2 + 0] = v[0];
+    ws[V.slot * 2 + 1] = v[1];
   }
 }
 
 // Norm of the 8-block RHS: max over |rhs_x|, |rhs_c|, |rhs_zl|, |rhs_zu|.
-__global__ void ir_rhs_norm_kernel(amigo_ipm_problem P, amigo_ipm_state S, double mu,
+__global__ void ir_rhs_norm_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S, double mu,
+                                   const double* __restrict__ mub,
                                    const double* __restrict__ grad,
                                    const double* __restrict__ rhs_saved, double* ws) {
+  const View V = view(B, P, S);
+  if (V.skip) return;
+  mu = pick(mu, mub, V);
+  grad += V.xo;
+  rhs_saved += V.xo;
   __shared__ double red[(NT / 64) * 1];
   double v[1] = {0.0};
   const int ops[1] = {MAX};
   const int total = P.n_primal + P.n_constraints;
-  for (int t = blockIdx.x * NT + threadIdx.x; t < total; t += gridDim.x * NT) {
+  for (int t = V.bx * NT + threadIdx.x; t < total; t += V.gx * NT) {
     if (t < P.n_primal) {
       const int i = t, idx = P.primal_indices[i];
       const bool fl = finite_bound(P.lbx[i]), fu = finite_bound(P.ubx[i]);
@@ -697,17 +898,26 @@ __global__ void ir_rhs_norm_kernel(amigo_ipm_problem P, amigo_ipm_state S, doubl
     }
   }
   block_reduce<1>(v, ops, red);
-  if (threadIdx.x == 0) ws[blockIdx.x] = v[0];
+  if (threadIdx.x == 0) ws[V.slot] = v[0];
 }
 
 // dzl += (e_zl - zl dc)/gl, dzu += (e_zu + zu dc)/gu, px += corr.
-__global__ void ir_update_kernel(amigo_ipm_problem P, amigo_ipm_state S, int n,
+__global__ void ir_update_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S, int n,
                                  const double* __restrict__ corr,
                                  const double* __restrict__ ezl,
                                  const double* __restrict__ ezu,
                                  double* __restrict__ px, double* __restrict__ dzl,
                                  double* __restrict__ dzu) {
-  for (int t = blockIdx.x * NT + threadIdx.x; t < n; t += gridDim.x * NT) {
+  const View V = view(B, P, S);
+  if (V.skip) return;
+  if (B.prob) n = V.n;
+  corr += V.xo;
+  px += V.xo;
+  ezl += V.p0;
+  ezu += V.p0;
+  dzl += V.p0;
+  dzu += V.p0;
+  for (int t = V.bx * NT + threadIdx.x; t < n; t += V.gx * NT) {
     if (t < P.n_primal) {
       const int i = t;
       const double dc = corr[P.primal_indices[i]];
@@ -719,10 +929,17 @@ __global__ void ir_update_kernel(amigo_ipm_problem P, amigo_ipm_state S, int n,
 }
 
 // Initial bound-dual steps from px (linear_solver.py:177-179).
-__global__ void ir_init_dz_kernel(amigo_ipm_problem P, amigo_ipm_state S, double mu,
+__global__ void ir_init_dz_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S, double mu,
+                                  const double* __restrict__ mub,
                                   const double* __restrict__ px,
                                   double* __restrict__ dzl, double* __restrict__ dzu) {
-  for (int i = blockIdx.x * NT + threadIdx.x; i < P.n_primal; i += gridDim.x * NT) {
+  const View V = view(B, P, S);
+  if (V.skip) return;
+  mu = pick(mu, mub, V);
+  px += V.xo;
+  dzl += V.p0;
+  dzu += V.p0;
+  for (int i = V.bx * NT + threadIdx.x; i < P.n_primal; i += V.gx * NT) {
     const double dx = px[P.primal_indices[i]];
     const bool fl = finite_bound(P.lbx[i]), fu = finite_bound(P.ubx[i]);
     const double gl = fl ? S.sl[i] : 1.0, gu = fu ? S.su[i] : 1.0;
@@ -731,16 +948,44 @@ __global__ void ir_init_dz_kernel(amigo_ipm_problem P, amigo_ipm_state S, double
   }
 }
 
-__global__ void scatter_scale_kernel(int nc, const int* __restrict__ cidx,
+__global__ void scatter_scale_kernel(BT B, int nc, const int* __restrict__ cidx,
                                      const double* __restrict__ dc,
                                      double* __restrict__ v, int mode) {
-  for (int j = blockIdx.x * NT + threadIdx.x; j < nc; j += gridDim.x * NT) {
+  amigo_ipm_problem P{0, nc, nullptr, cidx, nullptr, nullptr, dc};
+  amigo_ipm_state S0{nullptr, nullptr, nullptr, nullptr, nullptr};
+  const View V = view(B, P, S0);
+  if (V.skip) return;
+  nc = P.n_constraints;
+  cidx += V.c0;
+  dc += V.c0;
+  v += mode == 2 ? V.c0 : V.xo;
+  for (int j = V.bx * NT + threadIdx.x; j < nc; j += V.gx * NT) {
     if (mode == 0)
       v[cidx[j]] *= dc[j];
     else if (mode == 1)
       v[cidx[j]] /= dc[j];
     else
       v[j] *= dc[j];
+  }
+}
+
+// Copy of the active problems' segments: what = 1 -> x only, 2 -> zl, zu,
+// sl, su only, 3 -> all five vectors (OptVector::copy of the problems that
+// take part).  Null destination members are skipped.
+__global__ void copy_state_kernel(BT B, amigo_ipm_state D, amigo_ipm_state S, int what) {
+  amigo_ipm_problem P{0, 0, nullptr, nullptr, nullptr, nullptr, nullptr};
+  const View V = view(B, P, S);
+  if (V.skip) return;
+  shift_state(D, V);
+  const int np = B.pseg[V.k + 1] - V.p0;
+  for (int t = V.bx * NT + threadIdx.x; t < V.n; t += V.gx * NT) {
+    if (what & 1) D.xlam[t] = S.xlam[t];
+    if ((what & 2) && t < np) {
+      D.zl[t] = S.zl[t];
+      D.zu[t] = S.zu[t];
+      D.sl[t] = S.sl[t];
+      D.su[t] = S.su[t];
+    }
   }
 }
 
@@ -760,13 +1005,16 @@ const char* amigo_ipm_last_error(void) { return g_ipm_err.c_str(); }
 
 int amigo_ipm_workspace_doubles(void) { return MAXB * 4 + 16; }
 
+int amigo_ipm_grid_blocks(int n) { return nblocks(n); }
+
 int amigo_ipm_residual(const amigo_ipm_problem* P, const amigo_ipm_state* S,
                        double mu, const double* grad, double* res, double* ws,
                        double* out3, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   const int total = P->n_primal + P->n_constraints;
   const int nb = nblocks(total);
-  hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(NT), 0, s, *P, *S, mu, grad, res, ws);
+  hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(NT), 0, s, BT{}, *P, *S, mu, nullptr,
+                     grad, res, ws);
   LAUNCH_CHECK("residual_kernel");
   const int ops[3] = {SUM, SUM, SUM};
   return finish_and_copy(3, ops, ws, nb, out3, s);
@@ -775,7 +1023,8 @@ int amigo_ipm_residual(const amigo_ipm_problem* P, const amigo_ipm_state* S,
 int amigo_ipm_diagonal(const amigo_ipm_problem* P, const amigo_ipm_state* S,
                        double* diag, void* stream) {
   const int total = P->n_primal + P->n_constraints;
-  hipLaunchKernelGGL(diagonal_kernel, dim3(grid_for(total)), dim3(NT), 0, (hipStream_t)stream, *P, *S, diag);
+  hipLaunchKernelGGL(diagonal_kernel, dim3(grid_for(total)), dim3(NT), 0, (hipStream_t)stream,
+                     BT{}, *P, *S, diag);
   LAUNCH_CHECK("diagonal_kernel");
   return 0;
 }
@@ -784,7 +1033,7 @@ int amigo_ipm_bound_dual_step(const amigo_ipm_problem* P, const amigo_ipm_state*
                               double mu, const double* px, double* dzl, double* dzu,
                               void* stream) {
   hipLaunchKernelGGL(bound_dual_step_kernel, dim3(grid_for(P->n_primal)), dim3(NT), 0,
-                     (hipStream_t)stream, *P, *S, mu, px, dzl, dzu);
+                     (hipStream_t)stream, BT{}, *P, *S, mu, nullptr, px, dzl, dzu);
   LAUNCH_CHECK("bound_dual_step_kernel");
   return 0;
 }
@@ -794,9 +1043,10 @@ int amigo_ipm_max_step(const amigo_ipm_problem* P, const amigo_ipm_state* S,
                        const double* dzu, double* ws, double* out4, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   const int nb = nblocks(P->n_primal);
-  hipLaunchKernelGGL(max_step_kernel, dim3(nb), dim3(NT), 0, s, *P, *S, tau, px, dzl, dzu, ws);
+  hipLaunchKernelGGL(max_step_kernel, dim3(nb), dim3(NT), 0, s, BT{}, *P, *S, tau, nullptr,
+                     px, dzl, dzu, ws);
   double* out = ws + (size_t)MAXB * 4;
-  hipLaunchKernelGGL(max_step_finish, dim3(1), dim3(64), 0, s, *P, ws, nb, out);
+  hipLaunchKernelGGL(max_step_finish, dim3(1), dim3(64), 0, s, BT{}, 1, *P, ws, nb, out);
   LAUNCH_CHECK("max_step_kernel");
   hipError_t e = hipMemcpyAsync(out4, out, 4 * sizeof(double), hipMemcpyDeviceToHost, s);
   if (e != hipSuccess) return set_err("hipMemcpyAsync", e);
@@ -810,9 +1060,16 @@ int amigo_ipm_apply_step(const amigo_ipm_problem* P, const amigo_ipm_state* S,
                          const double* dzu, int n, const amigo_ipm_state* T,
                          void* stream) {
   hipLaunchKernelGGL(apply_step_kernel, dim3(grid_for(n)), dim3(NT), 0, (hipStream_t)stream,
-                     *P, *S, ax, az, dxlam, dzl, dzu, n, *T);
+                     BT{}, *P, *S, ax, az, nullptr, nullptr, dxlam, dzl, dzu, n, *T);
   LAUNCH_CHECK("apply_step_kernel");
   return 0;
+}
+
+static void reduce_ops(int kind, int* ops) {
+  ops[0] = ops[1] = ops[2] = ops[3] = SUM;
+  if (kind == RK_COMPL) ops[2] = MIN;
+  if (kind == RK_KKT_ERR) ops[0] = ops[1] = ops[2] = MAX;
+  if (kind == RK_MAX_COMP_DEV) ops[0] = MAX;
 }
 
 int amigo_ipm_reduce(int kind, const amigo_ipm_problem* P, const amigo_ipm_state* S,
@@ -821,12 +1078,11 @@ int amigo_ipm_reduce(int kind, const amigo_ipm_problem* P, const amigo_ipm_state
   hipStream_t s = (hipStream_t)stream;
   const int total = P->n_primal + P->n_constraints;
   const int nb = nblocks(total);
-  hipLaunchKernelGGL(reduce_kernel, dim3(nb), dim3(NT), 0, s, kind, *P, *S, mu, a, b, c, ws);
+  hipLaunchKernelGGL(reduce_kernel, dim3(nb), dim3(NT), 0, s, BT{}, kind, *P, *S, mu, nullptr,
+                     a, b, c, ws);
   LAUNCH_CHECK("reduce_kernel");
-  int ops[4] = {SUM, SUM, SUM, SUM};
-  if (kind == RK_COMPL) ops[2] = MIN;
-  if (kind == RK_KKT_ERR) ops[0] = ops[1] = ops[2] = MAX;
-  if (kind == RK_MAX_COMP_DEV) ops[0] = MAX;
+  int ops[4];
+  reduce_ops(kind, ops);
   return finish_and_copy(4, ops, ws, nb, out4, s);
 }
 
@@ -834,7 +1090,7 @@ int amigo_ipm_reset_bound_multipliers(const amigo_ipm_problem* P,
                                       const amigo_ipm_state* S, double mu,
                                       double kappa, void* stream) {
   hipLaunchKernelGGL(reset_multipliers_kernel, dim3(grid_for(P->n_primal)), dim3(NT), 0,
-                     (hipStream_t)stream, *P, *S, mu, kappa);
+                     (hipStream_t)stream, BT{}, *P, *S, mu, nullptr, kappa);
   LAUNCH_CHECK("reset_multipliers_kernel");
   return 0;
 }
@@ -843,8 +1099,10 @@ int amigo_ipm_dual_residual_vector(const amigo_ipm_problem* P,
                                    const amigo_ipm_state* S, const double* grad,
                                    double* out, int n, void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(dual_residual_kernel, dim3(grid_for(n)), dim3(NT), 0, s, *P, *S, grad, out, n);
-  hipLaunchKernelGGL(dual_residual_fill, dim3(grid_for(P->n_primal)), dim3(NT), 0, s, *P, *S, grad, out);
+  hipLaunchKernelGGL(dual_residual_kernel, dim3(grid_for(n)), dim3(NT), 0, s, BT{}, *P, *S,
+                     grad, out, n);
+  hipLaunchKernelGGL(dual_residual_fill, dim3(grid_for(P->n_primal)), dim3(NT), 0, s, BT{}, *P,
+                     *S, grad, out);
   LAUNCH_CHECK("dual_residual_kernel");
   return 0;
 }
@@ -854,8 +1112,8 @@ int amigo_ipm_initialize(const double* lbx0, const double* ubx0, int np,
                          double kappa2, double* lbx, double* ubx,
                          const amigo_ipm_state* S, int do_relax, int do_project,
                          void* stream) {
-  hipLaunchKernelGGL(init_kernel, dim3(grid_for(np)), dim3(NT), 0, (hipStream_t)stream, lbx0,
-                     ubx0, np, pidx, relax, cvt, kappa1, kappa2, lbx, ubx, *S, do_relax,
+  hipLaunchKernelGGL(init_kernel, dim3(grid_for(np)), dim3(NT), 0, (hipStream_t)stream, BT{},
+                     lbx0, ubx0, np, pidx, relax, cvt, kappa1, kappa2, lbx, ubx, *S, do_relax,
                      do_project);
   LAUNCH_CHECK("init_kernel");
   return 0;
@@ -867,8 +1125,8 @@ int amigo_ipm_floor_slacks(const amigo_ipm_problem* P, const amigo_ipm_state* S,
                            double* out1, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   const int nb = nblocks(P->n_primal);
-  hipLaunchKernelGGL(floor_slacks_kernel, dim3(nb), dim3(NT), 0, s, *P, *S, lbx0, ubx0, mu,
-                     s_min, slack_move, ws);
+  hipLaunchKernelGGL(floor_slacks_kernel, dim3(nb), dim3(NT), 0, s, BT{}, *P, *S, lbx0, ubx0,
+                     mu, nullptr, s_min, nullptr, slack_move, ws);
   LAUNCH_CHECK("floor_slacks_kernel");
   const int ops[1] = {SUM};
   return finish_and_copy(1, ops, ws, nb, out1, s);
@@ -912,7 +1170,7 @@ int amigo_ipm_constraint_scaling(int nc, const int* cidx, const int* rowp,
 int amigo_ipm_scale_rows(int nc, const int* cidx, const double* dc, double* v,
                          int mode, void* stream) {
   hipLaunchKernelGGL(scatter_scale_kernel, dim3(grid_for(nc)), dim3(NT), 0, (hipStream_t)stream,
-                     nc, cidx, dc, v, mode);
+                     BT{}, nc, cidx, dc, v, mode);
   LAUNCH_CHECK("scatter_scale_kernel");
   return 0;
 }
@@ -922,12 +1180,12 @@ int amigo_ipm_ir_init(const amigo_ipm_problem* P, const amigo_ipm_state* S, doub
                       double* dzl, double* dzu, double* ws, double* out1,
                       void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(ir_init_dz_kernel, dim3(grid_for(P->n_primal)), dim3(NT), 0, s, *P, *S,
-                     mu, px, dzl, dzu);
+  hipLaunchKernelGGL(ir_init_dz_kernel, dim3(grid_for(P->n_primal)), dim3(NT), 0, s, BT{}, *P,
+                     *S, mu, nullptr, px, dzl, dzu);
   const int total = P->n_primal + P->n_constraints;
   const int nb = nblocks(total);
-  hipLaunchKernelGGL(ir_rhs_norm_kernel, dim3(nb), dim3(NT), 0, s, *P, *S, mu, grad,
-                     rhs_saved, ws);
+  hipLaunchKernelGGL(ir_rhs_norm_kernel, dim3(nb), dim3(NT), 0, s, BT{}, *P, *S, mu, nullptr,
+                     grad, rhs_saved, ws);
   LAUNCH_CHECK("ir_init");
   const int ops[1] = {MAX};
   return finish_and_copy(1, ops, ws, nb, out1, s);
@@ -941,8 +1199,8 @@ int amigo_ipm_ir_residual(const amigo_ipm_problem* P, const amigo_ipm_state* S,
   hipStream_t s = (hipStream_t)stream;
   const int total = P->n_primal + P->n_constraints;
   const int nb = nblocks(total);
-  hipLaunchKernelGGL(ir_residual_kernel, dim3(nb), dim3(NT), 0, s, *P, *S, mu, grad,
-                     rhs_saved, Kpx, px, dzl, dzu, econd, ezl, ezu, ws);
+  hipLaunchKernelGGL(ir_residual_kernel, dim3(nb), dim3(NT), 0, s, BT{}, *P, *S, mu, nullptr,
+                     grad, rhs_saved, Kpx, px, dzl, dzu, econd, ezl, ezu, ws);
   LAUNCH_CHECK("ir_residual_kernel");
   const int ops[2] = {MAX, MAX};
   return finish_and_copy(2, ops, ws, nb, out2, s);
@@ -951,9 +1209,258 @@ int amigo_ipm_ir_residual(const amigo_ipm_problem* P, const amigo_ipm_state* S,
 int amigo_ipm_ir_update(const amigo_ipm_problem* P, const amigo_ipm_state* S, int n,
                         const double* corr, const double* ezl, const double* ezu,
                         double* px, double* dzl, double* dzu, void* stream) {
-  hipLaunchKernelGGL(ir_update_kernel, dim3(grid_for(n)), dim3(NT), 0, (hipStream_t)stream, *P,
-                     *S, n, corr, ezl, ezu, px, dzl, dzu);
+  hipLaunchKernelGGL(ir_update_kernel, dim3(grid_for(n)), dim3(NT), 0, (hipStream_t)stream,
+                     BT{}, *P, *S, n, corr, ezl, ezu, px, dzl, dzu);
   LAUNCH_CHECK("ir_update_kernel");
+  return 0;
+}
+
+// ---- batches -----------------------------------------------------------------
+
+static BT bt_total(const amigo_ipm_batch* B) {
+  return BT{B->blk_total, B->first_total, B->count_total, B->pseg, B->cseg, B->xseg, B->active};
+}
+static BT bt_primal(const amigo_ipm_batch* B) {
+  return BT{B->blk_primal, B->first_primal, B->count_primal, B->pseg, B->cseg, B->xseg,
+            B->active};
+}
+
+static int batch_ok(const amigo_ipm_batch* B) {
+  if (!B || B->num_problems < 1 || B->nblk_total < 1 || B->nblk_primal < 1 || !B->pseg ||
+      !B->cseg || !B->xseg || !B->blk_total || !B->first_total || !B->count_total ||
+      !B->blk_primal || !B->first_primal || !B->count_primal) {
+    g_ipm_err = "invalid batch description";
+    return -1;
+  }
+  return 0;
+}
+
+// Combine per-problem partials into out_dev[K*NV] and copy to the host.
+static int batch_finish(const amigo_ipm_batch* B, BT bt, int NV, const int* ops,
+                        const double* ws, double* out_dev, double* out_host, hipStream_t s) {
+  const int K = B->num_problems;
+  switch (NV) {
+    case 1:
+      hipLaunchKernelGGL(finish_batch_kernel<1>, dim3(K), dim3(NT), 0, s, bt, ws, ops[0], 0, 0,
+                         0, out_dev);
+      break;
+    case 2:
+      hipLaunchKernelGGL(finish_batch_kernel<2>, dim3(K), dim3(NT), 0, s, bt, ws, ops[0], ops[1],
+                         0, 0, out_dev);
+      break;
+    case 3:
+      hipLaunchKernelGGL(finish_batch_kernel<3>, dim3(K), dim3(NT), 0, s, bt, ws, ops[0], ops[1],
+                         ops[2], 0, out_dev);
+      break;
+    default:
+      hipLaunchKernelGGL(finish_batch_kernel<4>, dim3(K), dim3(NT), 0, s, bt, ws, ops[0], ops[1],
+                         ops[2], ops[3], out_dev);
+      break;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err("finish_batch_kernel", e);
+  if (out_host) {
+    e = hipMemcpyAsync(out_host, out_dev, sizeof(double) * NV * (size_t)K,
+                       hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return set_err("hipMemcpyAsync", e);
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return set_err("hipStreamSynchronize", e);
+  }
+  return 0;
+}
+
+int amigo_ipm_batch_residual(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                             const amigo_ipm_state* S, const double* mu, const double* grad,
+                             double* res, double* ws, double* out_dev, double* out3,
+                             void* stream) {
+  if (batch_ok(B)) return -1;
+  hipStream_t s = (hipStream_t)stream;
+  const BT bt = bt_total(B);
+  hipLaunchKernelGGL(residual_kernel, dim3(B->nblk_total), dim3(NT), 0, s, bt, *P, *S, 0.0, mu,
+                     grad, res, ws);
+  LAUNCH_CHECK("residual_kernel");
+  const int ops[3] = {SUM, SUM, SUM};
+  return batch_finish(B, bt, 3, ops, ws, out_dev, out3, s);
+}
+
+int amigo_ipm_batch_diagonal(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                             const amigo_ipm_state* S, double* diag, void* stream) {
+  if (batch_ok(B)) return -1;
+  hipLaunchKernelGGL(diagonal_kernel, dim3(B->nblk_total), dim3(NT), 0, (hipStream_t)stream,
+                     bt_total(B), *P, *S, diag);
+  LAUNCH_CHECK("diagonal_kernel");
+  return 0;
+}
+
+int amigo_ipm_batch_bound_dual_step(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                                    const amigo_ipm_state* S, const double* mu,
+                                    const double* px, double* dzl, double* dzu, void* stream) {
+  if (batch_ok(B)) return -1;
+  hipLaunchKernelGGL(bound_dual_step_kernel, dim3(B->nblk_primal), dim3(NT), 0,
+                     (hipStream_t)stream, bt_primal(B), *P, *S, 0.0, mu, px, dzl, dzu);
+  LAUNCH_CHECK("bound_dual_step_kernel");
+  return 0;
+}
+
+int amigo_ipm_batch_max_step(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                             const amigo_ipm_state* S, const double* tau, const double* px,
+                             const double* dzl, const double* dzu, double* ws,
+                             double* out_dev, double* out4, void* stream) {
+  if (batch_ok(B)) return -1;
+  hipStream_t s = (hipStream_t)stream;
+  const BT bt = bt_primal(B);
+  const int K = B->num_problems;
+  hipLaunchKernelGGL(max_step_kernel, dim3(B->nblk_primal), dim3(NT), 0, s, bt, *P, *S, 0.0,
+                     tau, px, dzl, dzu, ws);
+  hipLaunchKernelGGL(max_step_finish, dim3((K + 63) / 64), dim3(64), 0, s, bt, K, *P, ws, 0,
+                     out_dev);
+  LAUNCH_CHECK("max_step_kernel");
+  if (out4) {
+    hipError_t e = hipMemcpyAsync(out4, out_dev, 4 * sizeof(double) * (size_t)K,
+                                  hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return set_err("hipMemcpyAsync", e);
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return set_err("hipStreamSynchronize", e);
+  }
+  return 0;
+}
+
+int amigo_ipm_batch_apply_step(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                               const amigo_ipm_state* S, const double* ax, const double* az,
+                               const double* dxlam, const double* dzl, const double* dzu,
+                               const amigo_ipm_state* T, void* stream) {
+  if (batch_ok(B)) return -1;
+  hipLaunchKernelGGL(apply_step_kernel, dim3(B->nblk_total), dim3(NT), 0, (hipStream_t)stream,
+                     bt_total(B), *P, *S, 0.0, 0.0, ax, az, dxlam, dzl, dzu, 0, *T);
+  LAUNCH_CHECK("apply_step_kernel");
+  return 0;
+}
+
+int amigo_ipm_batch_reduce(int kind, const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                           const amigo_ipm_state* S, const double* mu, const double* a,
+                           const double* b, const double* c, double* ws, double* out_dev,
+                           double* out4, void* stream) {
+  if (batch_ok(B)) return -1;
+  hipStream_t s = (hipStream_t)stream;
+  const BT bt = bt_total(B);
+  hipLaunchKernelGGL(reduce_kernel, dim3(B->nblk_total), dim3(NT), 0, s, bt, kind, *P, *S, 0.0,
+                     mu, a, b, c, ws);
+  LAUNCH_CHECK("reduce_kernel");
+  int ops[4];
+  reduce_ops(kind, ops);
+  return batch_finish(B, bt, 4, ops, ws, out_dev, out4, s);
+}
+
+int amigo_ipm_batch_reset_bound_multipliers(const amigo_ipm_batch* B,
+                                            const amigo_ipm_problem* P,
+                                            const amigo_ipm_state* S, const double* mu,
+                                            double kappa, void* stream) {
+  if (batch_ok(B)) return -1;
+  hipLaunchKernelGGL(reset_multipliers_kernel, dim3(B->nblk_primal), dim3(NT), 0,
+                     (hipStream_t)stream, bt_primal(B), *P, *S, 0.0, mu, kappa);
+  LAUNCH_CHECK("reset_multipliers_kernel");
+  return 0;
+}
+
+int amigo_ipm_batch_dual_residual_vector(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                                         const amigo_ipm_state* S, const double* grad,
+                                         double* out, void* stream) {
+  if (batch_ok(B)) return -1;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(dual_residual_kernel, dim3(B->nblk_total), dim3(NT), 0, s, bt_total(B), *P,
+                     *S, grad, out, 0);
+  hipLaunchKernelGGL(dual_residual_fill, dim3(B->nblk_primal), dim3(NT), 0, s, bt_primal(B), *P,
+                     *S, grad, out);
+  LAUNCH_CHECK("dual_residual_kernel");
+  return 0;
+}
+
+int amigo_ipm_batch_initialize(const amigo_ipm_batch* B, const double* lbx0,
+                               const double* ubx0, const int* pidx, double relax, double cvt,
+                               double kappa1, double kappa2, double* lbx, double* ubx,
+                               const amigo_ipm_state* S, int do_relax, int do_project,
+                               void* stream) {
+  if (batch_ok(B)) return -1;
+  hipLaunchKernelGGL(init_kernel, dim3(B->nblk_primal), dim3(NT), 0, (hipStream_t)stream,
+                     bt_primal(B), lbx0, ubx0, 0, pidx, relax, cvt, kappa1, kappa2, lbx, ubx, *S,
+                     do_relax, do_project);
+  LAUNCH_CHECK("init_kernel");
+  return 0;
+}
+
+int amigo_ipm_batch_floor_slacks(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                                 const amigo_ipm_state* S, const double* lbx0,
+                                 const double* ubx0, const double* mu, const double* s_min,
+                                 double slack_move, double* ws, double* out_dev, double* out1,
+                                 void* stream) {
+  if (batch_ok(B)) return -1;
+  hipStream_t s = (hipStream_t)stream;
+  const BT bt = bt_primal(B);
+  hipLaunchKernelGGL(floor_slacks_kernel, dim3(B->nblk_primal), dim3(NT), 0, s, bt, *P, *S, lbx0,
+                     ubx0, 0.0, mu, 0.0, s_min, slack_move, ws);
+  LAUNCH_CHECK("floor_slacks_kernel");
+  const int ops[1] = {SUM};
+  return batch_finish(B, bt, 1, ops, ws, out_dev, out1, s);
+}
+
+int amigo_ipm_batch_scale_rows(const amigo_ipm_batch* B, const int* cidx, const double* dc,
+                               double* v, int mode, void* stream) {
+  if (batch_ok(B)) return -1;
+  hipLaunchKernelGGL(scatter_scale_kernel, dim3(B->nblk_total), dim3(NT), 0, (hipStream_t)stream,
+                     bt_total(B), 0, cidx, dc, v, mode);
+  LAUNCH_CHECK("scatter_scale_kernel");
+  return 0;
+}
+
+int amigo_ipm_batch_ir_init(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                            const amigo_ipm_state* S, const double* mu, const double* grad,
+                            const double* rhs_saved, const double* px, double* dzl, double* dzu,
+                            double* ws, double* out_dev, double* out1, void* stream) {
+  if (batch_ok(B)) return -1;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(ir_init_dz_kernel, dim3(B->nblk_primal), dim3(NT), 0, s, bt_primal(B), *P,
+                     *S, 0.0, mu, px, dzl, dzu);
+  const BT bt = bt_total(B);
+  hipLaunchKernelGGL(ir_rhs_norm_kernel, dim3(B->nblk_total), dim3(NT), 0, s, bt, *P, *S, 0.0,
+                     mu, grad, rhs_saved, ws);
+  LAUNCH_CHECK("ir_init");
+  const int ops[1] = {MAX};
+  return batch_finish(B, bt, 1, ops, ws, out_dev, out1, s);
+}
+
+int amigo_ipm_batch_ir_residual(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                                const amigo_ipm_state* S, const double* mu, const double* grad,
+                                const double* rhs_saved, const double* Kpx, const double* px,
+                                const double* dzl, const double* dzu, double* econd,
+                                double* ezl, double* ezu, double* ws, double* out_dev,
+                                double* out2, void* stream) {
+  if (batch_ok(B)) return -1;
+  hipStream_t s = (hipStream_t)stream;
+  const BT bt = bt_total(B);
+  hipLaunchKernelGGL(ir_residual_kernel, dim3(B->nblk_total), dim3(NT), 0, s, bt, *P, *S, 0.0,
+                     mu, grad, rhs_saved, Kpx, px, dzl, dzu, econd, ezl, ezu, ws);
+  LAUNCH_CHECK("ir_residual_kernel");
+  const int ops[2] = {MAX, MAX};
+  return batch_finish(B, bt, 2, ops, ws, out_dev, out2, s);
+}
+
+int amigo_ipm_batch_ir_update(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                              const amigo_ipm_state* S, const double* corr, const double* ezl,
+                              const double* ezu, double* px, double* dzl, double* dzu,
+                              void* stream) {
+  if (batch_ok(B)) return -1;
+  hipLaunchKernelGGL(ir_update_kernel, dim3(B->nblk_total), dim3(NT), 0, (hipStream_t)stream,
+                     bt_total(B), *P, *S, 0, corr, ezl, ezu, px, dzl, dzu);
+  LAUNCH_CHECK("ir_update_kernel");
+  return 0;
+}
+
+int amigo_ipm_batch_copy_state(const amigo_ipm_batch* B, const amigo_ipm_state* dst,
+                               const amigo_ipm_state* src, int what, void* stream) {
+  if (batch_ok(B)) return -1;
+  hipLaunchKernelGGL(copy_state_kernel, dim3(B->nblk_total), dim3(NT), 0, (hipStream_t)stream,
+                     bt_total(B), *dst, *src, what);
+  LAUNCH_CHECK("copy_state_kernel");
   return 0;
 }
 

@@ -45,6 +45,7 @@ struct DevState {
   unsigned long long* tops = nullptr;  // [2] bump pointers (rpool, ipool)
   unsigned long long rcap = 0, icap = 0;
   int* flags = nullptr;    // [8]: 0 error, 1..3 inertia pos/neg/zero
+  int* ninert = nullptr;   // [K*4] per-node pivot signs: pos, neg, zero, non-finite
 };
 
 struct FactorParams {
@@ -82,6 +83,20 @@ class LdlNumeric {
     *zero = inertia_[2];
   }
   int pivot_counts(int* np, int* nd, std::string& err) const;
+  // Independent diagonal blocks (e.g. many problems in one block-diagonal
+  // KKT): node_seg[b]..node_seg[b+1] are block b's nodes.  After each
+  // factorization the inertia is also summed per block, and a non-finite
+  // pivot is reported per block instead of failing the factorization.
+  int set_batch(int nblocks, const int* node_seg, std::string& err);
+  int batch_blocks() const { return nbatch_; }
+  void batch_inertia(int* pos, int* neg, int* zero, int* nan) const {
+    for (int b = 0; b < nbatch_; b++) {
+      if (pos) pos[b] = binert_[4 * b + 0];
+      if (neg) neg[b] = binert_[4 * b + 1];
+      if (zero) zero[b] = binert_[4 * b + 2];
+      if (nan) nan[b] = binert_[4 * b + 3];
+    }
+  }
   void stats(int* delayed, int* max_front) const {
     *delayed = delayed_;
     *max_front = max_front_seen_;
@@ -201,6 +216,10 @@ class LdlNumeric {
   double delay_growth_ = 2.0;
   unsigned long long base_rcap_ = 0, base_icap_ = 0;
   int inertia_[3] = {0, 0, 0};
+  int nbatch_ = 0;
+  int* d_batch_seg_ = nullptr;      // [nbatch+1]
+  int* d_batch_inert_ = nullptr;    // [nbatch*4]
+  std::vector<int> binert_;
   double flops_ = 0.0, solve_bytes_ = 0.0;
   bool factored_ = false;
   int max_p_ = 0;                   // largest fully summed block seen

@@ -702,6 +702,7 @@ __global__ __launch_bounds__(NT) void level_kernel(
       S.pstate[PS * node + 2] = 0;
       S.pstate[PS * node + 3] = 0;
       S.pstate[PS * node + 4] = 1;
+      for (int q = 0; q < 4; q++) S.ninert[4 * node + q] = 0;
     }
     return;
   }
@@ -808,6 +809,7 @@ __global__ __launch_bounds__(NT) void level_kernel(
     if (cnt[1]) atomicAdd(&S.flags[2], cnt[1]);
     if (cnt[2]) atomicAdd(&S.flags[3], cnt[2]);
     if (cnt[3]) atomicAdd(&S.flags[4], cnt[3]);
+    for (int q = 0; q < 4; q++) S.ninert[4 * node + q] = cnt[q];
   }
 }
 
@@ -1445,6 +1447,8 @@ __global__ __launch_bounds__(REG ? NTH : NTP) void panel_kernel(DevState S,
     if (cnt[1]) atomicAdd(&S.flags[2], cnt[1]);
     if (cnt[2]) atomicAdd(&S.flags[3], cnt[2]);
     if (cnt[3]) atomicAdd(&S.flags[4], cnt[3]);
+    for (int q = 0; q < 4; q++)
+      if (cnt[q]) atomicAdd(&S.ninert[4 * node + q], cnt[q]);
   }
 }
 
@@ -1579,6 +1583,26 @@ __global__ void finalize_big_kernel(DevState S, const int* __restrict__ bnodes,
   S.ndel[node] = nfs - nc;
   if (nfs - nc) atomicAdd(&S.flags[5], nfs - nc);
   atomicMax(&S.flags[6], S.m[node]);
+}
+
+// Per-block inertia of a block-diagonal matrix: block b owns nodes
+// seg[b]..seg[b+1]; one thread per block sums its nodes' pivot signs in
+// node order (integer sums: exact).
+__global__ void batch_inertia_kernel(int nb, const int* __restrict__ seg,
+                                     const int* __restrict__ ninert, int* __restrict__ out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+  for (int k = seg[b]; k < seg[b + 1]; k++) {
+    c0 += ninert[4 * k + 0];
+    c1 += ninert[4 * k + 1];
+    c2 += ninert[4 * k + 2];
+    c3 += ninert[4 * k + 3];
+  }
+  out[4 * b + 0] = c0;
+  out[4 * b + 1] = c1;
+  out[4 * b + 2] = c2;
+  out[4 * b + 3] = c3;
 }
 
 // Symmetric equilibration by Ruiz iteration: each pass replaces
@@ -2118,6 +2142,8 @@ LdlNumeric::~LdlNumeric() {
   if (S_.fpos) (void)hipFree(S_.fpos);
   if (W_) (void)hipFree(W_);
   if (ub_) (void)hipFree(ub_);
+  if (d_batch_seg_) (void)hipFree(d_batch_seg_);
+  if (d_batch_inert_) (void)hipFree(d_batch_inert_);
 }
 
 void LdlNumeric::tk_begin(hipStream_t s) {
@@ -2380,6 +2406,7 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   if (dalloc(allocs_, K, &d_sync_ns_, err)) return -2;
   if (dalloc(allocs_, 2, &S_.tops, err)) return -2;
   if (dalloc(allocs_, 8, &S_.flags, err)) return -2;
+  if (dalloc(allocs_, 4 * (size_t)K, &S_.ninert, err)) return -2;
 
   {
     static bool attr_done = false;
@@ -2624,7 +2651,16 @@ int LdlNumeric::finish_factor(hipStream_t stream, std::string& err) {
   HIPCHK(hipMemcpyAsync(flags, S_.flags, sizeof(flags), hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));
   if (flags[0] != 0) return flags[0];
-  if (flags[4] != 0) {
+  if (nbatch_ > 0) {
+    // Per-block signs; a non-finite pivot fails only its own block.
+    hipLaunchKernelGGL(batch_inertia_kernel, dim3((nbatch_ + 255) / 256), dim3(256), 0, stream,
+                       nbatch_, (const int*)d_batch_seg_, (const int*)S_.ninert, d_batch_inert_);
+    HIPCHK(hipGetLastError());
+    binert_.resize(4 * (size_t)nbatch_);
+    HIPCHK(hipMemcpyAsync(binert_.data(), d_batch_inert_, 4 * sizeof(int) * nbatch_,
+                          hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  } else if (flags[4] != 0) {
     err = "non-finite pivot (the matrix holds NaN or Inf)";
     return -6;
   }
@@ -2858,6 +2894,27 @@ void LdlNumeric::capture_solve(int nrhs, double* x, int ldx, hipStream_t stream)
   }
   (void)hipGraphDestroy(g);
   (void)hipGetLastError();
+}
+
+int LdlNumeric::set_batch(int nblocks, const int* node_seg, std::string& err) {
+  if (nblocks < 1 || node_seg[0] != 0 || node_seg[nblocks] != K_) {
+    err = "block segments must cover the nodes";
+    return -1;
+  }
+  if (d_batch_seg_) {
+    (void)hipFree(d_batch_seg_);
+    d_batch_seg_ = nullptr;
+  }
+  if (d_batch_inert_) {
+    (void)hipFree(d_batch_inert_);
+    d_batch_inert_ = nullptr;
+  }
+  HIPCHK(hipMalloc(&d_batch_seg_, sizeof(int) * (nblocks + 1)));
+  HIPCHK(hipMalloc(&d_batch_inert_, sizeof(int) * 4 * (size_t)nblocks));
+  HIPCHK(hipMemcpy(d_batch_seg_, node_seg, sizeof(int) * (nblocks + 1), hipMemcpyHostToDevice));
+  nbatch_ = nblocks;
+  binert_.assign(4 * (size_t)nblocks, 0);
+  return 0;
 }
 
 int LdlNumeric::pivot_counts(int* np, int* nd, std::string& err) const {

@@ -873,12 +873,18 @@ void contribution_rows(int n, const int* colp, const int* rows,
 
 // SparseLDL::symbolic_analysis (sparse_ldl.h:2204-2358).
 void symbolic_analysis(int order, int n, const int* rowp, const int* cols,
-                       SymbolicPlan& P, const int* groups) {
+                       SymbolicPlan& P, const int* groups, const int* user_perm) {
   P.n = n;
   P.order = order;
   const int* perm = nullptr;
   const int* iperm = nullptr;
-  if (order != ORDER_NATURAL) {
+  if (user_perm) {
+    P.perm.assign(user_perm, user_perm + n);
+    P.iperm.assign(n, 0);
+    for (int i = 0; i < n; i++) P.iperm[P.perm[i]] = i;
+    perm = P.perm.data();
+    iperm = P.iperm.data();
+  } else if (order != ORDER_NATURAL) {
     compute_ordering(order, n, rowp, cols, P.perm, P.iperm, groups);
     perm = P.perm.data();
     iperm = P.iperm.data();
@@ -942,6 +948,55 @@ void symbolic_analysis(int order, int n, const int* rowp, const int* cols,
     P.cholesky_int_nnz += w;
     P.cholesky_factor_nnz += w * (w + c);
   }
+}
+
+void concat_plans(const std::vector<const SymbolicPlan*>& blocks,
+                  const std::vector<int>& var_offset, SymbolicPlan& P) {
+  int n = 0, K = 0;
+  size_t nc = 0;
+  for (const SymbolicPlan* b : blocks) {
+    n += b->n;
+    K += b->num_nodes;
+    nc += b->contrib_rows.size();
+  }
+  P.n = n;
+  P.order = blocks.empty() ? ORDER_NATURAL : blocks[0]->order;
+  P.num_nodes = K;
+  P.perm.assign(n, 0);
+  P.iperm.assign(n, 0);
+  P.node_ptr.assign(K + 1, 0);
+  P.node_to_var.assign(n, 0);
+  P.node_parent.assign(K, -1);
+  P.contrib_ptr.assign(K + 1, 0);
+  P.contrib_rows.assign(nc, 0);
+  P.max_contrib = 0;
+  P.cholesky_factor_nnz = 0;
+  P.cholesky_int_nnz = 0;
+  int k0 = 0;
+  size_t c0 = 0;
+  for (size_t b = 0; b < blocks.size(); b++) {
+    const SymbolicPlan& B = *blocks[b];
+    const int v0 = var_offset[b];
+    for (int i = 0; i < B.n; i++) {
+      P.perm[v0 + i] = v0 + (B.perm.empty() ? i : B.perm[i]);
+      P.node_to_var[v0 + i] = v0 + B.node_to_var[i];
+    }
+    for (int k = 0; k < B.num_nodes; k++) {
+      P.node_ptr[k0 + k + 1] = v0 + B.node_ptr[k + 1];
+      P.node_parent[k0 + k] = B.node_parent[k] >= 0 ? k0 + B.node_parent[k] : -1;
+      P.contrib_ptr[k0 + k + 1] = int(c0) + B.contrib_ptr[k + 1];
+    }
+    for (size_t q = 0; q < B.contrib_rows.size(); q++) P.contrib_rows[c0 + q] = v0 + B.contrib_rows[q];
+    P.max_contrib = std::max(P.max_contrib, B.max_contrib);
+    P.cholesky_factor_nnz += B.cholesky_factor_nnz;
+    P.cholesky_int_nnz += B.cholesky_int_nnz;
+    k0 += B.num_nodes;
+    c0 += B.contrib_rows.size();
+  }
+  for (int i = 0; i < n; i++) P.iperm[P.perm[i]] = i;
+  P.children_ptr.assign(K + 2, 0);
+  P.children.assign(K, 0);
+  child_lists(K, P.node_parent.data(), P.children_ptr.data(), P.children.data());
 }
 
 }  // namespace amigo_amd

@@ -73,7 +73,20 @@ struct SymbolicPlan {
 
 // Full symbolic analysis as in SparseLDL's constructor.  The CSR must hold
 // the full symmetric pattern with sorted rows.
+// With `user_perm` (perm[new] = old) the ordering step is skipped and that
+// permutation is used instead (e.g. the concatenated orderings of the
+// independent blocks of a block-diagonal matrix).
 void symbolic_analysis(int order, int n, const int* rowp, const int* cols,
-                       SymbolicPlan& plan, const int* groups = nullptr);
+                       SymbolicPlan& plan, const int* groups = nullptr,
+                       const int* user_perm = nullptr);
+
+// Plan of a block-diagonal matrix from the plans of its blocks (block b's
+// variables start at var_offset[b]): nodes, variables, contribution rows and
+// permutations are concatenated with offsets and the children lists rebuilt,
+// so every block keeps exactly the nodes of its own analysis.  (Analysing
+// the whole matrix at once need not: the relaxed amalgamation visits
+// children in an order keyed by their slot in the global child array.)
+void concat_plans(const std::vector<const SymbolicPlan*>& blocks,
+                  const std::vector<int>& var_offset, SymbolicPlan& out);
 
 }  // namespace amigo_amd

@@ -120,6 +120,8 @@ class SparseLDL:
         precision="fp64",
         groups=None,
         scaling=False,
+        perm=None,
+        _handle=None,
     ):
         self.mat = mat
         self.n = mat.nrows
@@ -132,20 +134,40 @@ class SparseLDL:
         if groups is not None:
             self._groups = np.ascontiguousarray(groups, dtype=np.int32)
             gp = _lib.ip(self._groups)
-        _lib.check(
-            _lib.lib().amigo_ldl_create_grouped(
-                self.n,
-                _lib.ip(mat.rowp),
-                _lib.ip(mat.cols),
-                1 if self.solver_type == CHOLESKY else 0,
-                float(ustab),
-                float(pivot_tol),
-                float(delay_growth),
-                _order_code(order),
-                gp,
-                ctypes.byref(h),
+        if _handle is not None:
+            h = _handle
+        elif perm is not None:
+            # Caller's permutation (perm[new] = old) in place of the ordering.
+            self._perm = np.ascontiguousarray(perm, dtype=np.int32)
+            _lib.check(
+                _lib.lib().amigo_ldl_create_perm(
+                    self.n,
+                    _lib.ip(mat.rowp),
+                    _lib.ip(mat.cols),
+                    1 if self.solver_type == CHOLESKY else 0,
+                    float(ustab),
+                    float(pivot_tol),
+                    float(delay_growth),
+                    _lib.ip(self._perm),
+                    ctypes.byref(h),
+                )
             )
-        )
+        if _handle is None and perm is None:
+            _lib.check(
+                _lib.lib().amigo_ldl_create_grouped(
+                    self.n,
+                    _lib.ip(mat.rowp),
+                    _lib.ip(mat.cols),
+                    1 if self.solver_type == CHOLESKY else 0,
+                    float(ustab),
+                    float(pivot_tol),
+                    float(delay_growth),
+                    _order_code(order),
+                    gp,
+                    ctypes.byref(h),
+                )
+            )
+        self._nbatch = 0
         self._h = h
         self._factored = False
         if scaling:
@@ -177,6 +199,45 @@ class SparseLDL:
         _lib.check(_lib.lib().amigo_ldl_get_symbolic(self._h, *[_lib.ip(a) for a in arrs]))
         names = ["node_ptr", "node_to_var", "children_ptr", "children", "contrib_ptr", "contrib_rows", "invperm"]
         return dict(zip(names, arrs))
+
+    @classmethod
+    def union(cls, mat, blocks, precision="fp64", scaling=False):
+        """SparseLDL of the block-diagonal union of the blocks' matrices (mat
+        holds that union pattern): the blocks' symbolic plans are
+        concatenated, so each block keeps its own fronts."""
+        arr = (ctypes.c_void_p * len(blocks))(*[b._h.value for b in blocks])
+        h = ctypes.c_void_p()
+        _lib.check(
+            _lib.lib().amigo_ldl_create_union(
+                mat.nrows, _lib.ip(mat.rowp), _lib.ip(mat.cols), len(blocks), arr, ctypes.byref(h)
+            )
+        )
+        b0 = blocks[0]
+        return cls(mat, b0.solver_type, precision=precision, scaling=scaling, _handle=h)
+
+    def get_perm(self):
+        """The symmetric permutation (perm[new] = old) of the analysis."""
+        return np.argsort(self.get_symbolic()["invperm"]).astype(np.int32)
+
+    # ---- block-diagonal batches -----------------------------------------
+    def set_batch(self, node_seg):
+        """Independent diagonal blocks: block b owns nodes node_seg[b] ..
+        node_seg[b+1] (contiguous).  Each factorization then also reports
+        the inertia per block, and a non-finite pivot is confined to its
+        block (get_batch_inertia) instead of failing the factorization."""
+        seg = np.ascontiguousarray(node_seg, dtype=np.int32)
+        _lib.check(_lib.lib().amigo_ldl_set_batch(self._h, len(seg) - 1, _lib.ip(seg)))
+        self._nbatch = len(seg) - 1
+
+    def get_batch_inertia(self):
+        """(npos, nneg, nzero, nnan) arrays, one entry per block; npos counts
+        zero pivots as positive, as get_inertia does."""
+        B = self._nbatch
+        a, b, c, d = (np.zeros(B, np.int32) for _ in range(4))
+        _lib.check(
+            _lib.lib().amigo_ldl_get_batch_inertia(self._h, _lib.ip(a), _lib.ip(b), _lib.ip(c), _lib.ip(d))
+        )
+        return a, b, c, d
 
     # ---- numeric --------------------------------------------------------
     def factor(self, shift=None, precision=None):

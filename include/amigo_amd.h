@@ -82,6 +82,27 @@ int amigo_ldl_create_grouped(int n, const int* rowp, const int* cols,
                              int solver_type, double ustab, double pivot_tol,
                              double delay_growth, int order,
                              const int* var_group, amigo_ldl** out);
+/* Same, with the symmetric permutation given (perm[new] = old) instead of
+ * computed: the OrderingUtils::reorder step of SparseLDL's constructor
+ * (include/sparse_ldl.h:108-117, ordering_utils.h:38-56) is replaced by the
+ * caller's permutation; the rest of the symbolic analysis
+ * (sparse_ldl.h:2204-2358) is unchanged.  Used for block-diagonal unions
+ * of independent KKT matrices, whose permutation is the concatenation of
+ * each block's own ordering, so that every block keeps the fronts it has
+ * alone. */
+int amigo_ldl_create_perm(int n, const int* rowp, const int* cols, int solver_type,
+                          double ustab, double pivot_tol, double delay_growth,
+                          const int* perm, amigo_ldl** out);
+/* Handle for the block-diagonal union of independent KKT matrices whose
+ * own handles (same solver parameters) are given in order: block b's
+ * variables follow those of blocks 0..b-1, and (rowp, cols) must be the
+ * blocks' patterns placed on the diagonal.  The symbolic plan is the
+ * concatenation of the blocks' plans (SparseLDL's constructor,
+ * include/sparse_ldl.h:65-118, run once per block), so every block is
+ * factored with the fronts it has alone while all blocks share each level
+ * kernel launch.  The block handles may be destroyed afterwards. */
+int amigo_ldl_create_union(int n, const int* rowp, const int* cols, int num_blocks,
+                           const amigo_ldl* const* blocks, amigo_ldl** out);
 int amigo_ldl_destroy(amigo_ldl* h);
 
 /* Sizes of the symbolic arrays: num_nodes, total contribution rows, number
@@ -116,6 +137,18 @@ int amigo_ldl_factor_shifted(amigo_ldl* h, const double* values_dev,
  * pivots as positive like the reference; nzero reports them separately. */
 int amigo_ldl_get_inertia(const amigo_ldl* h, int* npos, int* nneg,
                           int* nzero);
+
+/* Block-diagonal batches (many independent KKT matrices factored as one):
+ * node_seg (host, num_blocks+1 entries, from 0 to num_nodes) gives each
+ * block's contiguous node range.  After every factorization the inertia is
+ * also counted per block (SparseLDL::get_inertia of each block's own
+ * factor, include/sparse_ldl.h:198-206, :1917-1969), and a non-finite pivot
+ * is reported in that block's nnan count instead of failing the whole
+ * factorization.  get_batch_inertia takes host arrays of num_blocks ints
+ * (any may be NULL). */
+int amigo_ldl_set_batch(amigo_ldl* h, int num_blocks, const int* node_seg);
+int amigo_ldl_get_batch_inertia(const amigo_ldl* h, int* npos, int* nneg, int* nzero,
+                                int* nnan);
 
 /* Per-node pivot and delayed-pivot counts of the last factorization
  * (MatrixFactor::get_factor, include/sparse_ldl.h:411-445). */

@@ -20,6 +20,10 @@ enum {
   AMIGO_ELEM_CARTPOLE = 4     /* cart-pole collocation interval */
 };
 
+/* First input of a padding element (a batch pads each problem's elements
+ * to whole blocks of 256). */
+#define AMIGO_ELEM_PAD (-2)
+
 const char* amigo_assembly_last_error(void);
 
 /* Number of inputs per element of a type (-1 if unknown). */
@@ -33,6 +37,16 @@ int amigo_elements_eval(int type, int E, const int* idx, const double* data,
                         int ndata, const double* x, double alpha, double* gbuf,
                         double* hbuf, double* lpart, void* stream);
 
+/* Same, for a batch of problems laid end to end with each problem's
+ * elements padded to whole blocks of 256: block b scales the objective by
+ * alpha_blk[b] (device, one value per block; NULL = alpha for all), so each
+ * problem keeps its own objective scale (the obj_scale of
+ * InteriorPointOptimizer::compute_nlp_scaling, optimizer.h:502-563). */
+int amigo_elements_eval_scaled(int type, int E, const int* idx, const double* data,
+                               int ndata, const double* x, double alpha,
+                               const double* alpha_blk, double* gbuf, double* hbuf,
+                               double* lpart, void* stream);
+
 /* out[i] (+)= sum over p in [ptr[i], ptr[i+1]) of buf[src[p]], summed in
  * list order (deterministic). */
 int amigo_gather_sum(int n, const int* ptr, const long long* src,
@@ -42,6 +56,13 @@ int amigo_gather_sum(int n, const int* ptr, const long long* src,
 /* out_dev[0] = sum of part[0..nb) in order; copied to out_host if given. */
 int amigo_sum_partials(const double* part, int nb, double* out_dev,
                        double* out_host, void* stream);
+
+/* Per-problem Lagrangians of a batch (OptimizationProblem::lagrangian,
+ * optimization_problem.h:718-733, per problem): out_dev[k] = sum of
+ * part[lidx[p]] for p in lptr[k] .. lptr[k+1] (device arrays), in list
+ * order; copied to out_host (K doubles) if given. */
+int amigo_sum_partials_batch(const double* part, int K, const int* lptr, const int* lidx,
+                             double* out_dev, double* out_host, void* stream);
 
 #ifdef __cplusplus
 }

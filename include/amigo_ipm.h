@@ -149,6 +149,105 @@ int amigo_ipm_ir_update(const amigo_ipm_problem* P, const amigo_ipm_state* S,
                         const double* ezu, double* px, double* dzl, double* dzu,
                         void* stream);
 
+/* ---------------- batches of independent problems ----------------------
+ *
+ * K problems laid end to end in one set of vectors (no reference
+ * counterpart: the reference solves one problem per InteriorPointOptimizer).
+ * Problem k owns primal entries pseg[k]..pseg[k+1], constraint entries
+ * cseg[k]..cseg[k+1] and x-length entries xseg[k]..xseg[k+1]; its
+ * primal/constraint index lists hold the problem's own variable numbers.
+ * The block tables list, for every block of a launch, the problem it
+ * serves (blk_*), and per problem its first block and block count
+ * (first_*, count_*); count_total[k] must be
+ * amigo_ipm_grid_blocks(np_k + nc_k) and count_primal[k]
+ * amigo_ipm_grid_blocks(np_k), the grids the problem gets alone, so that
+ * each problem's results (reductions included) are bitwise those of the
+ * single-problem entry points.  `active` (K ints, NULL = all) selects the
+ * problems a call touches; the others' data are left as they are.  Scalars
+ * (mu, tau, step lengths) are device arrays of K values.  Reductions write
+ * K rows into out_dev (device, K x 4 doubles) and copy them to the host
+ * array out* when it is non-NULL.  ws holds 4 x max(nblk_total,
+ * nblk_primal) doubles.  All arrays in the struct are device arrays. */
+typedef struct {
+  int num_problems;
+  const int* pseg;
+  const int* cseg;
+  const int* xseg;
+  int nblk_total;
+  const int* blk_total;
+  const int* first_total;
+  const int* count_total;
+  int nblk_primal;
+  const int* blk_primal;
+  const int* first_primal;
+  const int* count_primal;
+  const int* active;
+} amigo_ipm_batch;
+
+/* Blocks of the single-problem grid over n entries. */
+int amigo_ipm_grid_blocks(int n);
+
+/* Batched counterparts of the entry points above (same formulas and
+ * reference functions, per problem). */
+int amigo_ipm_batch_residual(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                             const amigo_ipm_state* S, const double* mu, const double* grad,
+                             double* res, double* ws, double* out_dev, double* out3,
+                             void* stream);
+int amigo_ipm_batch_diagonal(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                             const amigo_ipm_state* S, double* diag, void* stream);
+int amigo_ipm_batch_bound_dual_step(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                                    const amigo_ipm_state* S, const double* mu,
+                                    const double* px, double* dzl, double* dzu, void* stream);
+int amigo_ipm_batch_max_step(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                             const amigo_ipm_state* S, const double* tau, const double* px,
+                             const double* dzl, const double* dzu, double* ws,
+                             double* out_dev, double* out4, void* stream);
+int amigo_ipm_batch_apply_step(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                               const amigo_ipm_state* S, const double* ax, const double* az,
+                               const double* dxlam, const double* dzl, const double* dzu,
+                               const amigo_ipm_state* T, void* stream);
+int amigo_ipm_batch_reduce(int kind, const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                           const amigo_ipm_state* S, const double* mu, const double* a,
+                           const double* b, const double* c, double* ws, double* out_dev,
+                           double* out4, void* stream);
+int amigo_ipm_batch_reset_bound_multipliers(const amigo_ipm_batch* B,
+                                            const amigo_ipm_problem* P,
+                                            const amigo_ipm_state* S, const double* mu,
+                                            double kappa, void* stream);
+int amigo_ipm_batch_dual_residual_vector(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                                         const amigo_ipm_state* S, const double* grad,
+                                         double* out, void* stream);
+int amigo_ipm_batch_initialize(const amigo_ipm_batch* B, const double* lbx0,
+                               const double* ubx0, const int* pidx, double relax, double cvt,
+                               double kappa1, double kappa2, double* lbx, double* ubx,
+                               const amigo_ipm_state* S, int do_relax, int do_project,
+                               void* stream);
+int amigo_ipm_batch_floor_slacks(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                                 const amigo_ipm_state* S, const double* lbx0,
+                                 const double* ubx0, const double* mu, const double* s_min,
+                                 double slack_move, double* ws, double* out_dev, double* out1,
+                                 void* stream);
+int amigo_ipm_batch_scale_rows(const amigo_ipm_batch* B, const int* cidx, const double* dc,
+                               double* v, int mode, void* stream);
+int amigo_ipm_batch_ir_init(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                            const amigo_ipm_state* S, const double* mu, const double* grad,
+                            const double* rhs_saved, const double* px, double* dzl, double* dzu,
+                            double* ws, double* out_dev, double* out1, void* stream);
+int amigo_ipm_batch_ir_residual(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                                const amigo_ipm_state* S, const double* mu, const double* grad,
+                                const double* rhs_saved, const double* Kpx, const double* px,
+                                const double* dzl, const double* dzu, double* econd,
+                                double* ezl, double* ezu, double* ws, double* out_dev,
+                                double* out2, void* stream);
+int amigo_ipm_batch_ir_update(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
+                              const amigo_ipm_state* S, const double* corr, const double* ezl,
+                              const double* ezu, double* px, double* dzl, double* dzu,
+                              void* stream);
+/* OptVector::copy (include/interior_point_optimizer.h:21-90) of the active
+ * problems' segments: what = 1 x only, 2 zl/zu/sl/su only, 3 all. */
+int amigo_ipm_batch_copy_state(const amigo_ipm_batch* B, const amigo_ipm_state* dst,
+                               const amigo_ipm_state* src, int what, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

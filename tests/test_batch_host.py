@@ -1,0 +1,160 @@
+"""Host logic of the batched driver (CPU): the vectorized inertia corrector
+and filters against the single-problem classes they restate, the union
+problem's pattern / gather lists, and the union symbolic plan against each
+problem's own analysis."""
+
+import numpy as np
+import pytest
+
+from amigo_amd.batch import BatchFilter, BatchIC, BatchProblem, problem_ldl
+from amigo_amd.optimizer import Filter, InertiaCorrector
+from amigo_amd.problems import cart_pole, double_integrator
+from amigo_amd.sparse import CSRMat, SparseLDL
+
+
+def _ic_state(c):
+    return (c._delta_x_last, c._delta_c_last, c._delta_x_curr, c._delta_c_curr, c._hess_degen,
+            c._jac_degen, c._degen_iters, c._test_status, c._pivtol)
+
+
+def _bic_state(b, k):
+    return (b.dx_last[k], b.dc_last[k], b.dx_curr[k], b.dc_curr[k], b.hess_degen[k], b.jac_degen[k],
+            b.degen_iters[k], b.test[k], b.pivtol[k])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_batch_ic_matches_single_problem_ic(seed):
+    rng = np.random.default_rng(seed)
+    K = 7
+    mus = rng.uniform(1e-9, 1.0, K)
+    singles = [InertiaCorrector(np.zeros(1, bool), np.zeros(1, bool), mus[k], {}) for k in range(K)]
+    b = BatchIC(K, mus)
+    for step in range(300):
+        I = np.sort(rng.choice(K, size=rng.integers(1, K + 1), replace=False))
+        op = rng.integers(0, 5)
+        if op == 0:
+            b.consider_new_system(I)
+            for k in I:
+                singles[k]._consider_new_system()
+        elif op == 1:
+            r = b.perturb_wrong_inertia(I)
+            for t, k in enumerate(I):
+                assert r[t] == singles[k]._perturb_for_wrong_inertia()
+        elif op == 2:
+            r = b.perturb_singularity(I)
+            for t, k in enumerate(I):
+                assert r[t] == singles[k]._perturb_for_singularity()
+        elif op == 3:
+            r = b.increase_quality(I)
+            for t, k in enumerate(I):
+                assert r[t] == singles[k]._increase_quality(None)
+        else:
+            new = rng.uniform(1e-9, 1.0, K)
+            b.mu[I] = new[I]
+            for k in I:
+                singles[k].update_barrier(new[k])
+        for k in range(K):
+            assert _bic_state(b, k) == _ic_state(singles[k]), (step, op, k)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_batch_filter_matches_filter(seed):
+    rng = np.random.default_rng(seed)
+    K = 5
+    singles = [Filter() for _ in range(K)]
+    b = BatchFilter(K, cap=2)
+    for step in range(200):
+        I = np.sort(rng.choice(K, size=rng.integers(1, K + 1), replace=False))
+        phi = rng.normal(size=len(I))
+        th = rng.uniform(0, 2, size=len(I))
+        op = rng.integers(0, 4)
+        if op < 2:
+            b.add(I, phi, th)
+            for t, k in enumerate(I):
+                singles[k].add(phi[t], th[t])
+        elif op == 2:
+            got = b.acceptable(I, phi, th)
+            for t, k in enumerate(I):
+                assert got[t] == singles[k].is_acceptable(phi[t], th[t])
+        else:
+            b.clear(I[:1])
+            singles[I[0]].clear()
+        sz = b.size()
+        for k in range(K):
+            assert sz[k] == len(singles[k])
+            ent = sorted(zip(b.phi[k][b.valid[k]], b.theta[k][b.valid[k]]))
+            assert ent == sorted(singles[k].entries)
+
+
+def _problems():
+    return ([cart_pole.seeded(s, nmin=16, nmax=64, device="cpu") for s in range(4)]
+            + [double_integrator.build(N=20, device="cpu")])
+
+
+def test_batch_problem_is_block_diagonal_union():
+    from scipy.sparse import block_diag, csr_matrix
+
+    probs = _problems()
+    bp = BatchProblem(probs)
+    blocks = [csr_matrix((np.ones(p.nnz), p.cols, p.rowp), shape=(p.n, p.n)) for p in probs]
+    Bd = block_diag(blocks, format="csr")
+    Bd.sort_indices()
+    assert np.array_equal(Bd.indptr, bp.rowp) and np.array_equal(Bd.indices, bp.cols)
+    rng = np.random.default_rng(0)
+    for k, p in enumerate(probs):
+        # Gather lists: the same sources in the same order, remapped.
+        hb = rng.standard_normal(p.hsize)
+        ub = np.zeros(bp.hsize)
+        ub[bp._remap(k, p, np.arange(p.hsize), 2)] = hb
+        off = bp.rowp[bp.xseg[k]]
+        for i in range(p.nnz):
+            a = hb[p.h_src[p.h_ptr[i]:p.h_ptr[i + 1]]]
+            c = ub[bp.h_src[bp.h_ptr[off + i]:bp.h_ptr[off + i + 1]]]
+            assert np.array_equal(a, c)
+        gb = rng.standard_normal(p.gsize)
+        ug = np.zeros(bp.gsize)
+        ug[bp._remap(k, p, np.arange(p.gsize), 1)] = gb
+        x0 = bp.xseg[k]
+        for i in range(p.n):
+            assert np.array_equal(gb[p.g_src[p.g_ptr[i]:p.g_ptr[i + 1]]],
+                                  ug[bp.g_src[bp.g_ptr[x0 + i]:bp.g_ptr[x0 + i + 1]]])
+        assert np.array_equal(np.where(p.diag_idx >= 0, p.diag_idx + off, -1), bp.diag_idx[x0:x0 + p.n])
+        # Lagrangian partials: one block per 256 elements of each group.
+        assert bp.lptr[k + 1] - bp.lptr[k] == sum((g.E + 255) // 256 for g in p.groups)
+
+
+@pytest.mark.parametrize("order", ["ND", "AMD"])
+def test_union_plan_keeps_each_problems_fronts(order):
+    probs = _problems()
+    bp = BatchProblem(probs)
+    m = CSRMat(bp.n, bp.n, bp.rowp, bp.cols, None, device="cpu")
+    U = SparseLDL.union(m, [problem_ldl(p, order) for p in probs]).get_symbolic()
+    k0 = 0
+    for k, p in enumerate(probs):
+        mk = CSRMat(p.n, p.n, p.rowp, p.cols, None, device="cpu")
+        S = SparseLDL(mk, "LDL", order=order, groups=p.var_groups).get_symbolic()
+        nk = len(S["node_ptr"]) - 1
+        x0 = bp.xseg[k]
+        assert np.array_equal(U["node_ptr"][k0:k0 + nk + 1] - x0, S["node_ptr"])
+        assert np.array_equal(U["node_to_var"][x0:x0 + p.n] - x0, S["node_to_var"])
+        cp = U["contrib_ptr"][k0:k0 + nk + 1]
+        assert np.array_equal(cp - cp[0], S["contrib_ptr"])
+        assert np.array_equal(U["contrib_rows"][cp[0]:cp[-1]] - x0, S["contrib_rows"])
+        assert np.array_equal(U["invperm"][x0:x0 + p.n] - x0, S["invperm"])
+        # Children of every node, in the same order.
+        for j in range(nk):
+            cu = U["children"][U["children_ptr"][k0 + j]:U["children_ptr"][k0 + j + 1]] - k0
+            ca = S["children"][S["children_ptr"][j]:S["children_ptr"][j + 1]]
+            assert np.array_equal(cu, ca)
+        k0 += nk
+    assert k0 == len(U["node_ptr"]) - 1
+
+
+def test_union_rejects_mismatched_pattern():
+    from amigo_amd._lib import AmigoError
+
+    probs = _problems()[:2]
+    bp = BatchProblem(probs)
+    m = CSRMat(bp.n, bp.n, bp.rowp, bp.cols, None, device="cpu")
+    with pytest.raises(AmigoError):
+        SparseLDL.union(m, [problem_ldl(p, "ND") for p in probs[::-1]])
