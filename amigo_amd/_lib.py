@@ -106,6 +106,7 @@ _SIGS = {
         ],
     ),
     "amigo_ldl_set_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_int_p]),
+    "amigo_ldl_set_batch_mask": (ctypes.c_int, [ctypes.c_void_p, _c_int_p, ctypes.c_void_p]),
     "amigo_ldl_get_batch_inertia": (
         ctypes.c_int,
         [ctypes.c_void_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p],

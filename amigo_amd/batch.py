@@ -321,12 +321,13 @@ class BatchSolver:
     inertia counted per problem."""
 
     def __init__(self, bp, order="ND", ustab=0.01, pivot_tol=1e-14, delay_growth=2.0,
-                 scaling=False):
+                 scaling=False, precision="fp64"):
         self.bp = bp
         self.hess = bp.create_matrix()
         self.nrows = bp.n
         blocks = [problem_ldl(p, order, ustab, pivot_tol, delay_growth) for p in bp.problems]
-        self.ldl = SparseLDL.union(self.hess, blocks, scaling=scaling)
+        self.ldl = SparseLDL.union(self.hess, blocks, precision=precision, scaling=scaling)
+        self.precision = precision
         sym = self.ldl.get_symbolic()
         first = sym["node_to_var"][sym["node_ptr"][:-1]]
         owner = np.searchsorted(bp.xseg, first, side="right") - 1
@@ -334,32 +335,56 @@ class BatchSolver:
             raise RuntimeError("batch symbolic analysis interleaved the problems' nodes")
         self.node_seg = np.searchsorted(owner, np.arange(bp.K + 1), side="left").astype(np.int32)
         self.ldl.set_batch(self.node_seg)
+        self._mask_all = True
+        self.n_factor_problems = 0
         self.shift = torch.zeros(self.nrows, dtype=torch.float64, device=bp.device)
         self._tmp = torch.zeros(self.nrows, dtype=torch.float64, device=bp.device)
         self.n_factor = 0
         self.n_solve = 0
 
-    def factor(self, scaled, x, diag, post_hessian=None):
+    def _mask(self, mask):
+        key = None if mask is None or bool(np.all(mask)) else np.asarray(mask, bool)
+        if key is None and self._mask_all:
+            return
+        self.ldl.set_batch_mask(key)
+        self._mask_all = key is None
+
+    def factor(self, scaled, x, diag, post_hessian=None, mask=None):
         self.bp.hessian(scaled, x, self.hess)
         if post_hessian is not None:
             post_hessian(self.hess)
         self.shift.copy_(diag)
+        self._mask(mask)
         self.ldl.factor(shift=self.shift)
         self.n_factor += 1
 
     def assemble_hessian(self, scaled, x):
         self.bp.hessian(scaled, x, self.hess)
 
-    def add_diagonal_and_factor(self, diag):
+    def add_diagonal_and_factor(self, diag, mask=None):
+        """Factor the problems of mask (all when None); the others keep
+        their last factor."""
         self.shift.copy_(diag)
+        self._mask(mask)
         self.ldl.factor(shift=self.shift)
+        self.n_factor += 1
+        self.n_factor_problems += int(self.bp.K if mask is None else np.count_nonzero(mask))
+
+    def refactor_fp64(self, mask):
+        """FP64 factorization of the masked problems' current matrices (the
+        mixed-precision form's fallback)."""
+        self._mask(mask)
+        self.ldl.factor(shift=self.shift, precision="fp64")
         self.n_factor += 1
 
     def inertia(self):
         return self.ldl.get_batch_inertia()
 
-    def solve(self, bx, px):
+    def solve(self, bx, px, mask=None):
+        """px = K^-1 bx for the problems of mask; the others' entries of px
+        are bx's."""
         px.copy_(bx)
+        self._mask(mask)
         self.ldl.solve(px)
         self.n_solve += 1
 
@@ -909,15 +934,17 @@ class BatchOptimizer:
     Every per-problem decision follows optimizer.Optimizer.optimize; each
     device call covers all problems still in the phase that issues it."""
 
-    def __init__(self, problems, order="ND", scaling=True, verbose=False):
+    def __init__(self, problems, order="ND", scaling=True, verbose=False, precision="fp64"):
         self.problems = list(problems)
         self.order = order
         self.scaling = scaling
+        self.precision = precision
+        self.fp64_fallbacks = 0
         self.verbose = verbose
         t = time.perf_counter()
         bp = self.bp = BatchProblem(self.problems)
         self.K = bp.K
-        self.solver = BatchSolver(bp, order=order, scaling=scaling)
+        self.solver = BatchSolver(bp, order=order, scaling=scaling, precision=precision)
         be = self.be = BatchIPM(bp, bp.lower, bp.upper)
         dev = bp.device
         self.x = torch.as_tensor(bp.x0.copy(), dtype=torch.float64, device=dev)
@@ -935,6 +962,18 @@ class BatchOptimizer:
         self.setup_time = time.perf_counter() - t
         self.fallback = np.zeros(self.K, bool)
         self.results = None
+        self.profile_phases = False
+        self.phase_times = {}
+
+    def _tick(self, name, t0):
+        """Accumulate device-synchronized wall time of a phase (when
+        profile_phases is set); returns the new start time."""
+        if not self.profile_phases:
+            return t0
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        self.phase_times[name] = self.phase_times.get(name, 0.0) + t - t0
+        return t
 
     # ---- helpers ----------------------------------------------------------
     @staticmethod
@@ -1123,7 +1162,7 @@ class BatchOptimizer:
             diag.copy_(reg)
             diag.add_(torch.where(self.primal_dev & (dx > 0), dx, torch.zeros_like(dx)))
             diag.sub_(torch.where(self.mult_dev & (dc > 0), dc, torch.zeros_like(dc)))
-            self.solver.add_diagonal_and_factor(diag)
+            self.solver.add_diagonal_and_factor(diag, pending)
             p, q, _, nan = self.solver.inertia()
             P = np.nonzero(pending)[0]
             singular = nan[P] > 0
@@ -1184,9 +1223,20 @@ class BatchOptimizer:
         be = self.be
         be.residual(self.mu, self.vars, self.grad, self.res, B)
         rhs = self.res.clone()
-        self.solver.solve(self.res, self._sol)
+        self.solver.solve(self.res, self._sol, B)
         be.copy_x(self.px, self._sol, B)
-        self._refine(B, rhs)
+        ratio = self._refine(B, rhs)
+        if self.precision == "mixed":
+            # Refinement that ends above the 1e-10 ratio: FP64 factor of
+            # those problems' matrices, a fresh solve and refinement again
+            # (solver.AmigoSolver.iterative_refinement, per problem).
+            F = B & ~(ratio < 1e-10)
+            if F.any():
+                self.solver.refactor_fp64(F)
+                self.fp64_fallbacks += int(F.sum())
+                self.solver.solve(rhs, self._sol, F)
+                be.copy_x(self.px, self._sol, F)
+                self._refine(F, rhs)
         be.update(self.mu, self.vars, self.px, self.update, B)
 
     def _refine(self, B, rhs, max_steps=10):
@@ -1201,6 +1251,7 @@ class BatchOptimizer:
         nrm_rhs = be.ir_init(self.mu, self.vars, self.grad, rhs, self.px, dzl, dzu, B)
         ratio_max, improve, min_steps, max_cond = 1e-10, 0.999999999, 1, 1e6
         ratio_old = np.full(self.K, 1e30)
+        final = np.zeros(self.K)
         R = B.copy()
         for step in range(max_steps):
             if not R.any():
@@ -1211,14 +1262,16 @@ class BatchOptimizer:
             denom = np.minimum(nrm_res, max_cond * nrm_rhs) + nrm_rhs
             with np.errstate(divide="ignore", invalid="ignore"):
                 ratio = np.where(denom > 0, nrm_resid / np.where(denom > 0, denom, 1.0), 0.0)
+            final[R] = ratio[R]
             stop = R & (ratio < ratio_max) & (step >= min_steps)
             stop |= R & ~stop & (step >= min_steps) & (ratio > improve * ratio_old)
             R &= ~stop
             ratio_old = np.where(R, ratio, ratio_old)
             if not R.any():
                 break
-            self.solver.solve(self.res, self.ir_corr)
+            self.solver.solve(self.res, self.ir_corr, R)
             be.ir_update(self.vars, self.ir_corr, ezl, ezu, self.px, dzl, dzu, R)
+        return final
 
     # ---- filter line search (filter_line_search.py:48-280) -------------------
     def _line_search(self, S, alpha_x, alpha_z, tau, options, use_soc, phi_cur, wd):
@@ -1369,7 +1422,7 @@ class BatchOptimizer:
                     c_soc = torch.where(sel, self.res + asoc * c_soc, c_soc)
                     be.copy_x(self.res, c_soc, act)
                     self._update_gradient(self.vars.x, act)
-                    self.solver.solve(self.res, self._sol)
+                    self.solver.solve(self.res, self._sol, act)
                     be.copy_x(self.px, self._sol, act)
                     be.update(self.mu, self.vars, self.px, self.update, act)
                     sax, _, saz, _ = be.max_step(self.tau, self.vars, self.update, act)
@@ -1507,8 +1560,13 @@ class BatchOptimizer:
         self.fallback[:] = False
         record_obj = options.get("record_objective", True)
         self.iterations_run = 0
+        self.active_history = []
         for i in range(max_iters):
+            tp = time.perf_counter()
+            if self.profile_phases:
+                torch.cuda.synchronize()
             A = active.copy()
+            self.active_history.append(int(A.sum()))
             res_norm = be.residual(self.mu, self.vars, self.grad, self.res, A)
             ic.mu = self.mu.copy()
             need = A & np.isnan(self.theta0)
@@ -1548,10 +1606,13 @@ class BatchOptimizer:
             # _barrier_step: barrier update, then the Newton direction.
             self._barrier_update(i, res_norm, options, A)
             ic.mu = self.mu.copy()
+            tp = self._tick("convergence+barrier", tp)
             fok = self._factorize(A, options)
+            tp = self._tick("factorize+IC", tp)
             B = A & fok
             if B.any():
                 self._solve_with_mu(B)
+            tp = self._tick("solve+refinement", tp)
             changed = A & (self.mu != before)
             if changed.any():
                 C = np.nonzero(changed)[0]
@@ -1581,6 +1642,7 @@ class BatchOptimizer:
                 ax = az = np.minimum(ax, az)
             Sx = np.nonzero(B)[0]
             alpha, li, accepted, frej = self._line_search_with_watchdog(Sx, ax, az, self.tau, options, use_soc)
+            tp = self._tick("line_search", tp)
             line_iters[B] = li[B]
             acc_m = B & accepted
             filt_rej[acc_m & frej] += 1
@@ -1612,6 +1674,7 @@ class BatchOptimizer:
                 full = (alpha == 1.0) | (line_iters == 1)
                 self.wd_short[nw & full] = 0
                 self.wd_short[nw & ~full] += 1
+            self._tick("step", tp)
         self.results = data
         for k in range(K):
             data[k]["inertia_history"] = list(ic.history[k])
@@ -1621,12 +1684,27 @@ class BatchOptimizer:
         self.batch_time = time.perf_counter() - start
         F = np.nonzero(self.fallback)[0]
         self._alone = {}
-        for k in F:
-            o = Optimizer(self.problems[k], order=self.order, scaling=self.scaling)
-            d = o.optimize(options)
-            d["batched"] = False
-            data[k] = d
-            self._alone[k] = o
+        if F.size:
+            # The problems that left the batch are solved again on their own,
+            # several at once on their own streams (no graph capture beside
+            # other threads' work).
+            import concurrent.futures as cf
+
+            def one(k):
+                stream = torch.cuda.Stream(self.bp.device)
+                with torch.cuda.stream(stream):
+                    o = Optimizer(self.problems[k], order=self.order, scaling=self.scaling,
+                                  precision=self.precision)
+                    o.solver.ldl.set_graphs(False)
+                    d = o.optimize(options)
+                    stream.synchronize()
+                return k, o, d
+
+            with cf.ThreadPoolExecutor(max_workers=min(8, F.size)) as ex:
+                for k, o, d in ex.map(one, [int(k) for k in F]):
+                    d["batched"] = False
+                    data[k] = d
+                    self._alone[k] = o
         self.total_time = time.perf_counter() - start
         return data
 

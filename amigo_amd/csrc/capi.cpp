@@ -332,6 +332,15 @@ int amigo_ldl_set_batch(amigo_ldl* h, int num_blocks, const int* node_seg) {
   return AMIGO_OK;
 }
 
+int amigo_ldl_set_batch_mask(amigo_ldl* h, const int* mask, void* stream) {
+  if (!h) return fail(AMIGO_ERR_ARG, "null handle");
+  if (h->num.batch_blocks() == 0) return fail(AMIGO_ERR_STATE, "no batch segments set");
+  std::string err;
+  int rc = h->num.set_batch_mask(mask, (hipStream_t)stream, err);
+  if (rc) return fail(rc, "set_batch_mask: " + err);
+  return AMIGO_OK;
+}
+
 int amigo_ldl_get_batch_inertia(const amigo_ldl* h, int* npos, int* nneg, int* nzero,
                                 int* nnan) {
   if (!h) return fail(AMIGO_ERR_ARG, "null handle");

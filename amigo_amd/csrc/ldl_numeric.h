@@ -46,6 +46,17 @@ struct DevState {
   unsigned long long rcap = 0, icap = 0;
   int* flags = nullptr;    // [8]: 0 error, 1..3 inertia pos/neg/zero
   int* ninert = nullptr;   // [K*4] per-node pivot signs: pos, neg, zero, non-finite
+  // Batches of independent blocks: block of every node, each block's own
+  // pool regions (bump pointers tops[2b], tops[2b+1]) and the mask of the
+  // blocks a factorization or solve touches (all ones = every block).  A
+  // block left out keeps its fronts and factor untouched.
+  int nblk = 0;
+  const int* node_blk = nullptr;
+  const int* blk_active = nullptr;
+  const unsigned long long* blk_rbase = nullptr;
+  const unsigned long long* blk_ibase = nullptr;
+  const unsigned long long* blk_rcap = nullptr;
+  const unsigned long long* blk_icap = nullptr;
 };
 
 struct FactorParams {
@@ -88,6 +99,10 @@ class LdlNumeric {
   // factorization the inertia is also summed per block, and a non-finite
   // pivot is reported per block instead of failing the factorization.
   int set_batch(int nblocks, const int* node_seg, std::string& err);
+  // Blocks that the following factorizations and solves touch (host array
+  // of nblocks ints, nonzero = on; null = all).  The others keep their
+  // factors.
+  int set_batch_mask(const int* mask, hipStream_t stream, std::string& err);
   int batch_blocks() const { return nbatch_; }
   void batch_inertia(int* pos, int* neg, int* zero, int* nan) const {
     for (int b = 0; b < nbatch_; b++) {
@@ -220,6 +235,30 @@ class LdlNumeric {
   int* d_batch_seg_ = nullptr;      // [nbatch+1]
   int* d_batch_inert_ = nullptr;    // [nbatch*4]
   std::vector<int> binert_;
+  std::vector<int> h_node_blk_, h_blk_active_;
+  std::vector<unsigned long long> blk_rcap_h_, blk_icap_h_;
+  int* d_node_blk_ = nullptr;
+  int* d_blk_active_ = nullptr;
+  unsigned long long* d_blk_layout_ = nullptr;  // rbase, ibase, rcap, icap (4 x nbatch)
+  unsigned long long* d_tops_batch_ = nullptr;
+  int layout_batch(std::string& err);
+  // Launch lists of the blocks a mask leaves on (compacted on the host so
+  // that left-out fronts cost no workgroups): level node lists with their
+  // LDS-bucket splits, and the blocked-path plan of those fronts.
+  bool mask_all_ = true;
+  std::vector<int> m_level_ptr_, m_level_split_, m_level_nodes_, m_big_ptr_;
+  std::vector<int> m_big_nodes_, m_big_root_, m_big_ns_;
+  std::vector<int64_t> m_big_woff_;
+  int* d_m_level_nodes_ = nullptr;
+  int* d_m_big_nodes_ = nullptr;
+  int* d_m_big_root_ = nullptr;
+  int* d_m_big_ns_ = nullptr;
+  int64_t* d_m_big_woff_ = nullptr;
+  std::vector<int> h_big_root_, h_big_ns_;
+  std::vector<int64_t> h_big_woff_;
+  std::vector<int> h_nptr_, h_nvars_;
+  int* d_row_blk_ = nullptr;        // [n] block of every row (batches)
+  bool masked() const { return nbatch_ > 0 && !mask_all_; }
   double flops_ = 0.0, solve_bytes_ = 0.0;
   bool factored_ = false;
   int max_p_ = 0;                   // largest fully summed block seen

@@ -35,6 +35,11 @@ constexpr int BIG_M = 128;       // static fronts above this use the blocked pat
 constexpr int TILE = 64;         // Schur-update tile
 constexpr int DCAP_BIG = 64;     // delayed pivots a blocked front is sized for
 constexpr int DPAN_BIG = 16;     // delayed pivots the static panel count covers
+// A front gets blocked-path launches in the static plan when DPLAN_BIG
+// delayed pivots would take it past the LDS limit; one that receives more
+// than that past the limit makes the factorization redo itself level-
+// synchronously (fronts near 64 pivots then cost no empty panel launches).
+constexpr int DPLAN_BIG = 16;
 constexpr int PS = 8;            // ints of blocked-path state per front
 constexpr int NTP = 1024;        // threads of the panel workgroup (16 wave64)
 constexpr int NBW = NB + 2;      // W columns: NB pivots, a 2x2 overhang, imax
@@ -592,6 +597,11 @@ __device__ __forceinline__ int child_row_pos(const DevPlan& P, int c, int dc,
   return s < ns ? s : s + D;
 }
 
+// A node of a block left out of this factorization / solve (batches).
+__device__ __forceinline__ bool node_on(const DevState& S, int node) {
+  return S.blk_active == nullptr || S.blk_active[S.node_blk[node]] != 0;
+}
+
 // One workgroup per front of the level: allocate, assemble, and (unless the
 // front is routed to the blocked path) factor.
 __global__ __launch_bounds__(NT) void level_kernel(
@@ -608,6 +618,7 @@ __global__ __launch_bounds__(NT) void level_kernel(
   __shared__ int rowpos[LDS_MAX_M];
   const int tid = threadIdx.x;
   const int node = lnodes[blockIdx.x];
+  if (!node_on(S, node)) return;
   const int ns = P.ns[node], cs = P.cs[node];
   const int kb = P.kptr[node], ke = P.kptr[node + 1];
 
@@ -633,11 +644,24 @@ __global__ __launch_bounds__(NT) void level_kernel(
       D = -1;
     }
     const unsigned long long need = (unsigned long long)m * (unsigned long long)m;
-    const unsigned long long fo = atomicAdd(&S.tops[0], need);
-    const unsigned long long vo = atomicAdd(&S.tops[1], (unsigned long long)m);
+    unsigned long long fo, vo;
+    bool over;
+    if (S.node_blk) {
+      // The block's own regions of the pools.
+      const int b = S.node_blk[node];
+      const unsigned long long fr = atomicAdd(&S.tops[2 * b], need);
+      const unsigned long long vr = atomicAdd(&S.tops[2 * b + 1], (unsigned long long)m);
+      over = fr + need > S.blk_rcap[b] || vr + m > S.blk_icap[b];
+      fo = S.blk_rbase[b] + fr;
+      vo = S.blk_ibase[b] + vr;
+    } else {
+      fo = atomicAdd(&S.tops[0], need);
+      vo = atomicAdd(&S.tops[1], (unsigned long long)m);
+      over = fo + need > S.rcap || vo + m > S.icap;
+    }
     if (D < 0) {
       sh[0] = -1;
-    } else if (fo + need > S.rcap || vo + m > S.icap) {
+    } else if (over) {
       atomicExch(&S.flags[0], -4);
       sh[0] = -1;
     } else {
@@ -821,7 +845,7 @@ __global__ __launch_bounds__(NT) void level_kernel(
 // (3) extend-add the children, one child index per launch (child order as in
 // sparse_ldl.h:1084-1152).
 __device__ __forceinline__ bool big_active(const DevState& S, int node) {
-  return S.pstate[PS * node + 4] == 1 && atomicAdd(&S.flags[0], 0) == 0;
+  return node_on(S, node) && S.pstate[PS * node + 4] == 1 && atomicAdd(&S.flags[0], 0) == 0;
 }
 
 __global__ __launch_bounds__(NT) void big_zero_kernel(DevPlan P, DevState S,
@@ -1270,6 +1294,7 @@ __global__ __launch_bounds__(REG ? NTH : NTP) void panel_kernel(DevState S,
   if (tid == 0) bad = atomicAdd(&S.flags[0], 0);
   __syncthreads();
   if (bad != 0) return;
+  if (!node_on(S, node)) return;
   int* ps = S.pstate + PS * node;
   if (ps[4] != 1) return;  // factored in a workgroup of the level kernel
   const int m = S.m[node];
@@ -1487,6 +1512,7 @@ __global__ __launch_bounds__(NT) void trailing_mfma_kernel(DevState S,
   if (threadIdx.x == 0) bad = atomicAdd(&S.flags[0], 0);
   __syncthreads();
   if (bad != 0) return;
+  if (!node_on(S, node)) return;
   const int* ps = S.pstate + PS * node;
   const int m = S.m[node];
   const int k = ps[0], kstart = ps[2], kw = ps[3];
@@ -1572,6 +1598,7 @@ __global__ void finalize_big_kernel(DevState S, const int* __restrict__ bnodes,
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= count || atomicAdd(&S.flags[0], 0) != 0) return;
   const int node = bnodes[t];
+  if (!node_on(S, node)) return;
   if (S.pstate[PS * node + 4] == 0) return;  // factored in a workgroup
   const int k = S.pstate[PS * node + 0], nc = S.pstate[PS * node + 1];
   if (k < nc) {
@@ -1616,11 +1643,15 @@ __global__ void equilibrate_kernel(int n, const int* __restrict__ rowp,
                                    const double* __restrict__ vals,
                                    const double* __restrict__ shift,
                                    const double* __restrict__ sin,
-                                   double* __restrict__ sout) {
+                                   double* __restrict__ sout,
+                                   const int* __restrict__ row_blk,
+                                   const int* __restrict__ blk_active) {
   const int lane = threadIdx.x & 63;
   const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int nw = (gridDim.x * blockDim.x) >> 6;
   for (int r = w; r < n; r += nw) {
+    // Rows of blocks left out keep their scale factors.
+    if (row_blk && !blk_active[row_blk[r]]) continue;
     double m = 0.0;
     const double sr = sin ? sin[r] : 1.0;
     for (int p = rowp[r] + lane; p < rowp[r + 1]; p += 64) {
@@ -1636,16 +1667,25 @@ __global__ void equilibrate_kernel(int n, const int* __restrict__ rowp,
 }
 
 __global__ void scale_vec_kernel(int n, int nrhs, int ldx, const double* __restrict__ sc,
-                                 double* __restrict__ x) {
+                                 double* __restrict__ x, const int* __restrict__ row_blk,
+                                 const int* __restrict__ blk_active) {
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < (size_t)n * nrhs;
        t += (size_t)gridDim.x * blockDim.x) {
     const int i = int(t % n), r = int(t / n);
+    if (row_blk && !blk_active[row_blk[i]]) continue;
     x[i + (size_t)r * ldx] *= sc[i];
   }
 }
 
 __global__ void reset_kernel(DevState S) {
-  if (threadIdx.x == 0) {
+  if (S.node_blk) {
+    // Only the blocks that take part start their regions afresh.
+    for (int b = threadIdx.x; b < S.nblk; b += blockDim.x)
+      if (S.blk_active[b]) {
+        S.tops[2 * b] = 0;
+        S.tops[2 * b + 1] = 0;
+      }
+  } else if (threadIdx.x == 0) {
     S.tops[0] = 0;
     S.tops[1] = 0;
   }
@@ -1662,6 +1702,7 @@ __global__ __launch_bounds__(NT) void fpos_kernel(DevPlan P, DevState S, int K) 
   // allocated: nothing to map.
   if (atomicAdd(&S.flags[0], 0) != 0) return;
   for (int node = blockIdx.x; node < K; node += gridDim.x) {
+    if (!node_on(S, node)) continue;
     const int ns = P.ns[node], D = S.D[node], nfs = ns + D;
     const int* vars = S.ipool + S.voff[node];
     int* fpos = S.fpos + S.voff[node];
@@ -1931,6 +1972,7 @@ __global__ __launch_bounds__(NT) void solve_fwd_kernel(DevPlan P, DevState S,
   double* red = Ls + PIVOT_TRI;
   __shared__ int kmeta[4][NT];  // per child: delayed, update rows, offset, cmap start
   const int node = lnodes[blockIdx.x];
+  if (!node_on(S, node)) return;
   const int p = S.npiv[node], m = S.m[node];
   const double* L = S.rpool + S.foff[node];
   const int* vars = S.ipool + S.voff[node];
@@ -2053,6 +2095,7 @@ __global__ __launch_bounds__(NT) void solve_bwd_kernel(DevState S,
   double* t = smem;  // pivots then the other rows
   double* Ls = smem + tcap;
   const int node = lnodes[blockIdx.x];
+  if (!node_on(S, node)) return;
   const int p = S.npiv[node], m = S.m[node];
   if (p == 0) return;
   const double* L = S.rpool + S.foff[node];
@@ -2144,6 +2187,13 @@ LdlNumeric::~LdlNumeric() {
   if (ub_) (void)hipFree(ub_);
   if (d_batch_seg_) (void)hipFree(d_batch_seg_);
   if (d_batch_inert_) (void)hipFree(d_batch_inert_);
+  if (d_node_blk_) (void)hipFree(d_node_blk_);
+  if (d_blk_active_) (void)hipFree(d_blk_active_);
+  if (d_blk_layout_) (void)hipFree(d_blk_layout_);
+  if (d_tops_batch_) (void)hipFree(d_tops_batch_);
+  for (void* q : {(void*)d_m_level_nodes_, (void*)d_m_big_nodes_, (void*)d_m_big_root_,
+                  (void*)d_m_big_ns_, (void*)d_m_big_woff_, (void*)d_row_blk_})
+    if (q) (void)hipFree(q);
 }
 
 void LdlNumeric::tk_begin(hipStream_t s) {
@@ -2183,6 +2233,8 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   n_ = plan.n;
   K_ = plan.num_nodes;
   delay_growth_ = delay_growth;
+  h_nptr_ = plan.node_ptr;
+  h_nvars_ = plan.node_to_var;
   const int K = K_, n = n_;
   const bool permuted = !plan.iperm.empty();
   auto pinv = [&](int v) { return permuted ? plan.iperm[v] : v; };
@@ -2289,7 +2341,7 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   for (int k = 0; k < K; k++) {
     int m0 = ns[k] + cs[k];
     max_front_ = std::max(max_front_, m0);
-    big[k] = m0 + DCAP_BIG > LDS_MAX_M ? 1 : 0;
+    big[k] = m0 + DPLAN_BIG > LDS_MAX_M ? 1 : 0;
     rneed += double(m0) * m0;
     ineed += m0;
     double fk = 0.0;
@@ -2379,6 +2431,9 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
     const int* t2 = nullptr;
     const int* t3 = nullptr;
     const int64_t* t4 = nullptr;
+    h_big_root_ = big_root;
+    h_big_ns_ = big_ns;
+    h_big_woff_ = big_woff;
     if (upload(allocs_, big_nodes_, &t1, err)) return -2;
     if (upload(allocs_, big_root, &t2, err)) return -2;
     if (upload(allocs_, big_ns, &t3, err)) return -2;
@@ -2494,36 +2549,44 @@ int LdlNumeric::enqueue_factor(const double* vals, const double* shift,
     for (int it = 0; it < kPasses; it++) {
       double* sout = ((kPasses - 1 - it) % 2 == 0) ? scale_ : scale_tmp_;
       hipLaunchKernelGGL(equilibrate_kernel, dim3(nb), dim3(NT), 0, stream, n_,
-                         d_rowp_, d_cols_, vals, shift, sin, sout);
+                         d_rowp_, d_cols_, vals, shift, sin, sout,
+                         (const int*)(masked() ? d_row_blk_ : nullptr),
+                         (const int*)d_blk_active_);
       sin = sout;
     }
     HIPCHK(hipGetLastError());
   }
   const int nlev = num_levels();
+  // With a mask, the compacted lists of the blocks that take part.
+  const bool ml = masked();
+  const std::vector<int>& LP = ml ? m_level_ptr_ : level_ptr_;
+  const std::vector<int>& LS = ml ? m_level_split_ : level_split_;
+  const int* LN = ml ? d_m_level_nodes_ : d_level_nodes_;
+  const std::vector<int>& BP = ml ? m_big_ptr_ : big_ptr_;
   for (int l = 0; l < nlev; l++) {
-    const int split = level_split_[l];
-    const int parts[2][3] = {{level_ptr_[l], split - level_ptr_[l], level_lds_small_[l]},
-                             {split, level_ptr_[l + 1] - split, level_lds_[l]}};
+    const int split = LS[l];
+    const int parts[2][3] = {{LP[l], split - LP[l], level_lds_small_[l]},
+                             {split, LP[l + 1] - split, level_lds_[l]}};
     tk_begin(stream);
     for (const auto& pt : parts) {
       if (pt[1] == 0) continue;
       const int lds = pt[2];
       hipLaunchKernelGGL(level_kernel, dim3(pt[1]), dim3(NT),
                          size_t(lds + 2 * LDS_MAX_M) * sizeof(double), stream, P_, S_, vals,
-                         shift, d_level_nodes_ + pt[0], d_big_mask_, lds,
+                         shift, LN + pt[0], d_big_mask_, lds,
                          prm.ustab, prm.pivot_tol, cholesky ? 1 : 0,
                          sync_mode ? -1 : DCAP_BIG,
                          (const double*)(scaling_ ? scale_ : nullptr));
     }
     tk_end(stream, TK_LEVEL);
     HIPCHK(hipGetLastError());
-    const int b0 = big_ptr_[l], nb = big_ptr_[l + 1] - big_ptr_[l];
+    const int b0 = BP[l], nb = BP[l + 1] - BP[l];
     if (nb == 0 && !sync_mode) continue;
     int npanels = big_npanels_[l], tiles = big_tiles_[l];
-    const int* bnodes = d_big_nodes_ + b0;
-    const int* broot = d_big_root_ + b0;
-    const int* bns = d_big_ns_ + b0;
-    const int64_t* woff = d_big_woff_ + b0;
+    const int* bnodes = (ml ? d_m_big_nodes_ : d_big_nodes_) + b0;
+    const int* broot = (ml ? d_m_big_root_ : d_big_root_) + b0;
+    const int* bns = (ml ? d_m_big_ns_ : d_big_ns_) + b0;
+    const int64_t* woff = (ml ? d_m_big_woff_ : d_big_woff_) + b0;
     int nlaunch = nb;
     if (sync_mode) {
       // Front sizes are known only after assembly: read them back and list
@@ -2546,6 +2609,7 @@ int LdlNumeric::enqueue_factor(const double* vals, const double* shift,
       tiles = 0;
       for (int t = level_ptr_[l]; t < level_ptr_[l + 1]; t++) {
         const int node = level_nodes_[t];
+        if (nbatch_ && !h_blk_active_[h_node_blk_[node]]) continue;
         if (hm[node] <= LDS_MAX_M) continue;
         ln.push_back(node);
         lr.push_back(h_cs_[node] == 0 ? 1 : 0);
@@ -2583,6 +2647,7 @@ int LdlNumeric::enqueue_factor(const double* vals, const double* shift,
       if (sync_mode) {
         for (int t = level_ptr_[l]; t < level_ptr_[l + 1]; t++) {
           const int node = level_nodes_[t];
+          if (nbatch_ && !h_blk_active_[h_node_blk_[node]]) continue;
           if (h_m_[node] > LDS_MAX_M)
             maxkids = std::max(maxkids, h_kptr_[node + 1] - h_kptr_[node]);
         }
@@ -2681,13 +2746,15 @@ int LdlNumeric::factor(const double* vals, const double* shift, double ustab,
   // Replay of the captured launch sequence (same buffers, stream and
   // parameters); a replay that needs the level-synchronous mode or larger
   // pools falls through to the eager path below.
-  const bool graph_ok = graphs_ && !timing_;
+  const bool graph_ok = graphs_ && !timing_ && !masked();
   if (graph_ok && fgraph_ && fg_vals_ == vals && fg_shift_ == shift && fg_stream_ == stream &&
       fg_ustab_ == ustab && fg_tol_ == pivot_tol && fg_chol_ == cholesky && fg_mixed_ == mixed) {
     HIPCHK(hipGraphLaunch(fgraph_, stream));
     const int rc = finish_factor(stream, err);
     if (rc == 0) {
-      max_p_ = max_front_seen_;
+      // A batch keeps the factors of the blocks a masked factorization
+      // left out: the solves' LDS sizing covers every factor kept.
+      max_p_ = nbatch_ ? std::max(max_p_, max_front_seen_) : max_front_seen_;
       factored_ = true;
       return 0;
     }
@@ -2700,6 +2767,23 @@ int LdlNumeric::factor(const double* vals, const double* shift, double ustab,
       // More delayed pivots than the static launch plan holds: redo the
       // factorization with per-level front sizes.
       sync_mode = true;
+      continue;
+    }
+    if (rc == -4 && nbatch_) {
+      // A block's delayed pivots outgrew its regions: double every
+      // region, re-lay the pools out and refactor every block (the
+      // re-layout loses the factors the mask kept).
+      for (int b = 0; b < nbatch_; b++) {
+        blk_rcap_h_[b] *= 2;
+        blk_icap_h_[b] *= 2;
+      }
+      int r2 = layout_batch(err);
+      if (r2) return r2;
+      std::fill(h_blk_active_.begin(), h_blk_active_.end(), 1);
+      mask_all_ = true;
+      HIPCHK(hipMemcpyAsync(d_blk_active_, h_blk_active_.data(), sizeof(int) * nbatch_,
+                            hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
       continue;
     }
     if (rc == -4) {
@@ -2716,8 +2800,8 @@ int LdlNumeric::factor(const double* vals, const double* shift, double ustab,
       return rc;
     }
     // Largest front of this factorization (reduced on the device): the
-    // solve kernels' LDS sizing.
-    max_p_ = max_front_seen_;
+    // solve kernels' LDS sizing (in a batch, of every factor kept).
+    max_p_ = nbatch_ ? std::max(max_p_, max_front_seen_) : max_front_seen_;
     factored_ = true;
     // Capture once the same buffers have been factored eagerly twice (a
     // handle factored once does not pay for a capture).
@@ -2752,7 +2836,8 @@ int LdlNumeric::solve(int nrhs, double* x, int ldx, hipStream_t stream,
     }
     ub_cap_ = ubneed;
   }
-  if (graphs_ && !timing_) {
+  const bool graph_ok = graphs_ && !timing_ && !masked();
+  if (graph_ok) {
     for (SolveGraph& g : sgraphs_) {
       if (g.x == x && g.nrhs == nrhs && g.ldx == ldx && g.stream == stream && g.maxp == max_p_) {
         HIPCHK(hipGraphLaunch(g.exec, stream));
@@ -2766,7 +2851,7 @@ int LdlNumeric::solve(int nrhs, double* x, int ldx, hipStream_t stream,
   if (timing_) {
     HIPCHK(hipStreamSynchronize(stream));
     tk_collect();
-  } else if (graphs_) {
+  } else if (graph_ok) {
     // Capture on the second eager solve of the same buffer.
     bool seen = false;
     for (const SolveGraph& g : sseen_)
@@ -2794,25 +2879,31 @@ int LdlNumeric::enqueue_solve(int nrhs, double* x, int ldx, hipStream_t stream,
   const int sb = std::min(4096, std::max(1, (n_ * std::max(nrhs, 1) + NT - 1) / NT));
   if (scaling_)
     hipLaunchKernelGGL(scale_vec_kernel, dim3(sb), dim3(NT), 0, stream, n_, nrhs, ldx,
-                       (const double*)scale_, x);
+                       (const double*)scale_, x, (const int*)(masked() ? d_row_blk_ : nullptr),
+                       (const int*)d_blk_active_);
+  const bool ml = masked();
+  const std::vector<int>& LP = ml ? m_level_ptr_ : level_ptr_;
+  const int* LN = ml ? d_m_level_nodes_ : d_level_nodes_;
   for (int l = 0; l < nlev; l++) {
-    const int cnt = level_ptr_[l + 1] - level_ptr_[l];
+    const int cnt = LP[l + 1] - LP[l];
+    if (cnt == 0) continue;
     tk_begin(stream);
     hipLaunchKernelGGL(solve_fwd_kernel, dim3(cnt), dim3(NT), lds, stream, P_, S_,
-                       d_level_nodes_ + level_ptr_[l], x, ldx, nrhs, ub_, (size_t)S_.icap,
-                       tcap);
+                       LN + LP[l], x, ldx, nrhs, ub_, (size_t)S_.icap, tcap);
     tk_end(stream, TK_SOLVE_FWD);
   }
   for (int l = nlev - 1; l >= 0; l--) {
-    const int cnt = level_ptr_[l + 1] - level_ptr_[l];
+    const int cnt = LP[l + 1] - LP[l];
+    if (cnt == 0) continue;
     tk_begin(stream);
     hipLaunchKernelGGL(solve_bwd_kernel, dim3(cnt), dim3(NT), lds, stream, S_,
-                       d_level_nodes_ + level_ptr_[l], x, ldx, nrhs, tcap);
+                       LN + LP[l], x, ldx, nrhs, tcap);
     tk_end(stream, TK_SOLVE_BWD);
   }
   if (scaling_)
     hipLaunchKernelGGL(scale_vec_kernel, dim3(sb), dim3(NT), 0, stream, n_, nrhs, ldx,
-                       (const double*)scale_, x);
+                       (const double*)scale_, x, (const int*)(masked() ? d_row_blk_ : nullptr),
+                       (const int*)d_blk_active_);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -2901,19 +2992,161 @@ int LdlNumeric::set_batch(int nblocks, const int* node_seg, std::string& err) {
     err = "block segments must cover the nodes";
     return -1;
   }
-  if (d_batch_seg_) {
-    (void)hipFree(d_batch_seg_);
-    d_batch_seg_ = nullptr;
-  }
-  if (d_batch_inert_) {
-    (void)hipFree(d_batch_inert_);
-    d_batch_inert_ = nullptr;
-  }
+  auto fr = [](auto*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+  };
+  fr(d_batch_seg_);
+  fr(d_batch_inert_);
+  fr(d_node_blk_);
+  fr(d_blk_active_);
   HIPCHK(hipMalloc(&d_batch_seg_, sizeof(int) * (nblocks + 1)));
   HIPCHK(hipMalloc(&d_batch_inert_, sizeof(int) * 4 * (size_t)nblocks));
   HIPCHK(hipMemcpy(d_batch_seg_, node_seg, sizeof(int) * (nblocks + 1), hipMemcpyHostToDevice));
   nbatch_ = nblocks;
   binert_.assign(4 * (size_t)nblocks, 0);
+  // Node -> block, and each block's pool needs at delay_growth x its
+  // static front sizes.
+  h_node_blk_.assign(K_, 0);
+  blk_rcap_h_.assign(nblocks, 0);
+  blk_icap_h_.assign(nblocks, 0);
+  for (int b = 0; b < nblocks; b++) {
+    double rn = 0.0, in = 0.0;
+    for (int k = node_seg[b]; k < node_seg[b + 1]; k++) {
+      h_node_blk_[k] = b;
+      const double m0 = double(h_ns_[k] + h_cs_[k]);
+      rn += m0 * m0;
+      in += m0;
+    }
+    blk_rcap_h_[b] = (unsigned long long)(std::max(1.0, delay_growth_) * rn) + 64;
+    blk_icap_h_[b] = (unsigned long long)(std::max(1.0, delay_growth_) * in) + 64;
+  }
+  h_blk_active_.assign(nblocks, 1);
+  mask_all_ = true;
+  {
+    std::vector<int> rb(n_, 0);
+    for (int k = 0; k < K_; k++)
+      for (int q = h_nptr_[k]; q < h_nptr_[k + 1]; q++) rb[h_nvars_[q]] = h_node_blk_[k];
+    fr(d_row_blk_);
+    HIPCHK(hipMalloc(&d_row_blk_, sizeof(int) * std::max(n_, 1)));
+    HIPCHK(hipMemcpy(d_row_blk_, rb.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
+  }
+  fr(d_m_level_nodes_);
+  fr(d_m_big_nodes_);
+  fr(d_m_big_root_);
+  fr(d_m_big_ns_);
+  fr(d_m_big_woff_);
+  HIPCHK(hipMalloc(&d_m_level_nodes_, sizeof(int) * std::max(K_, 1)));
+  const size_t nbig = std::max<size_t>(1, big_nodes_.size());
+  HIPCHK(hipMalloc(&d_m_big_nodes_, sizeof(int) * nbig));
+  HIPCHK(hipMalloc(&d_m_big_root_, sizeof(int) * nbig));
+  HIPCHK(hipMalloc(&d_m_big_ns_, sizeof(int) * nbig));
+  HIPCHK(hipMalloc(&d_m_big_woff_, sizeof(int64_t) * nbig));
+  HIPCHK(hipMalloc(&d_node_blk_, sizeof(int) * std::max(K_, 1)));
+  HIPCHK(hipMemcpy(d_node_blk_, h_node_blk_.data(), sizeof(int) * K_, hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&d_blk_active_, sizeof(int) * nblocks));
+  HIPCHK(hipMemcpy(d_blk_active_, h_blk_active_.data(), sizeof(int) * nblocks,
+                   hipMemcpyHostToDevice));
+  return layout_batch(err);
+}
+
+// Regions of every block in the pools (rpool doubles, ipool ints) from the
+// per-block capacities; grows the pools when they no longer hold them.
+int LdlNumeric::layout_batch(std::string& err) {
+  const int B = nbatch_;
+  std::vector<unsigned long long> lay(4 * (size_t)B);
+  unsigned long long r = 0, i = 0;
+  for (int b = 0; b < B; b++) {
+    lay[b] = r;
+    lay[B + b] = i;
+    lay[2 * B + b] = blk_rcap_h_[b];
+    lay[3 * B + b] = blk_icap_h_[b];
+    r += blk_rcap_h_[b];
+    i += blk_icap_h_[b];
+  }
+  int rc = ensure_pools(r, i, err);
+  if (rc) return rc;
+  if (d_blk_layout_) (void)hipFree(d_blk_layout_);
+  d_blk_layout_ = nullptr;
+  if (d_tops_batch_) (void)hipFree(d_tops_batch_);
+  d_tops_batch_ = nullptr;
+  HIPCHK(hipMalloc(&d_blk_layout_, sizeof(unsigned long long) * 4 * (size_t)B));
+  HIPCHK(hipMemcpy(d_blk_layout_, lay.data(), sizeof(unsigned long long) * 4 * (size_t)B,
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&d_tops_batch_, sizeof(unsigned long long) * 2 * (size_t)B));
+  HIPCHK(hipMemset(d_tops_batch_, 0, sizeof(unsigned long long) * 2 * (size_t)B));
+  S_.nblk = B;
+  S_.node_blk = d_node_blk_;
+  S_.blk_active = d_blk_active_;
+  S_.blk_rbase = d_blk_layout_;
+  S_.blk_ibase = d_blk_layout_ + B;
+  S_.blk_rcap = d_blk_layout_ + 2 * B;
+  S_.blk_icap = d_blk_layout_ + 3 * B;
+  S_.tops = d_tops_batch_;
+  // Kernel arguments changed: captured launch sequences are stale, and
+  // the re-laid-out pools hold no factor.
+  drop_graphs();
+  factored_ = false;
+  return 0;
+}
+
+int LdlNumeric::set_batch_mask(const int* mask, hipStream_t stream, std::string& err) {
+  if (!nbatch_) {
+    err = "no batch segments set";
+    return -3;
+  }
+  int on = 0;
+  for (int b = 0; b < nbatch_; b++) {
+    h_blk_active_[b] = mask ? (mask[b] != 0) : 1;
+    on += h_blk_active_[b];
+  }
+  mask_all_ = on == nbatch_;
+  HIPCHK(hipMemcpyAsync(d_blk_active_, h_blk_active_.data(), sizeof(int) * nbatch_,
+                        hipMemcpyHostToDevice, stream));
+  if (!mask_all_) {
+    // Compacted launch lists (same order, same LDS buckets).
+    const int nlev = num_levels();
+    m_level_ptr_.assign(nlev + 1, 0);
+    m_level_split_.assign(nlev, 0);
+    m_level_nodes_.clear();
+    m_big_ptr_.assign(nlev + 1, 0);
+    m_big_nodes_.clear();
+    m_big_root_.clear();
+    m_big_ns_.clear();
+    m_big_woff_.clear();
+    auto on_node = [&](int node) { return h_blk_active_[h_node_blk_[node]] != 0; };
+    for (int l = 0; l < nlev; l++) {
+      for (int t = level_ptr_[l]; t < level_split_[l]; t++)
+        if (on_node(level_nodes_[t])) m_level_nodes_.push_back(level_nodes_[t]);
+      m_level_split_[l] = int(m_level_nodes_.size());
+      for (int t = level_split_[l]; t < level_ptr_[l + 1]; t++)
+        if (on_node(level_nodes_[t])) m_level_nodes_.push_back(level_nodes_[t]);
+      m_level_ptr_[l + 1] = int(m_level_nodes_.size());
+      for (int t = big_ptr_[l]; t < big_ptr_[l + 1]; t++) {
+        if (!on_node(big_nodes_[t])) continue;
+        m_big_nodes_.push_back(big_nodes_[t]);
+        m_big_root_.push_back(h_big_root_[t]);
+        m_big_ns_.push_back(h_big_ns_[t]);
+        m_big_woff_.push_back(h_big_woff_[t]);
+      }
+      m_big_ptr_[l + 1] = int(m_big_nodes_.size());
+    }
+    if (!m_level_nodes_.empty())
+      HIPCHK(hipMemcpyAsync(d_m_level_nodes_, m_level_nodes_.data(),
+                            sizeof(int) * m_level_nodes_.size(), hipMemcpyHostToDevice, stream));
+    if (!m_big_nodes_.empty()) {
+      const size_t nb = m_big_nodes_.size();
+      HIPCHK(hipMemcpyAsync(d_m_big_nodes_, m_big_nodes_.data(), sizeof(int) * nb,
+                            hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(d_m_big_root_, m_big_root_.data(), sizeof(int) * nb,
+                            hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(d_m_big_ns_, m_big_ns_.data(), sizeof(int) * nb,
+                            hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(d_m_big_woff_, m_big_woff_.data(), sizeof(int64_t) * nb,
+                            hipMemcpyHostToDevice, stream));
+    }
+  }
+  HIPCHK(hipStreamSynchronize(stream));
   return 0;
 }
 

@@ -229,6 +229,15 @@ class SparseLDL:
         _lib.check(_lib.lib().amigo_ldl_set_batch(self._h, len(seg) - 1, _lib.ip(seg)))
         self._nbatch = len(seg) - 1
 
+    def set_batch_mask(self, mask=None):
+        """Blocks the following factorizations and solves touch (bool per
+        block; None = all).  Blocks left out keep their last factor."""
+        if mask is None:
+            _lib.check(_lib.lib().amigo_ldl_set_batch_mask(self._h, None, _lib.vp(current_stream_ptr())))
+            return
+        m = np.ascontiguousarray(np.asarray(mask, dtype=bool).astype(np.int32))
+        _lib.check(_lib.lib().amigo_ldl_set_batch_mask(self._h, _lib.ip(m), _lib.vp(current_stream_ptr())))
+
     def get_batch_inertia(self):
         """(npos, nneg, nzero, nnan) arrays, one entry per block; npos counts
         zero pivots as positive, as get_inertia does."""

@@ -149,6 +149,14 @@ int amigo_ldl_get_inertia(const amigo_ldl* h, int* npos, int* nneg,
 int amigo_ldl_set_batch(amigo_ldl* h, int num_blocks, const int* node_seg);
 int amigo_ldl_get_batch_inertia(const amigo_ldl* h, int* npos, int* nneg, int* nzero,
                                 int* nnan);
+/* Blocks that the following factorizations and solves touch (host array of
+ * num_blocks ints, nonzero = on; NULL = every block).  Each block has its
+ * own regions of the front pools, so a block that is left out keeps its
+ * last factor: a factorization refactors only the blocks that need it (an
+ * inertia-correction retry, the problems still iterating) and a solve runs
+ * only theirs.  Blocks' inertia counts are those of their last
+ * factorization.  Stream-ordered on `stream`. */
+int amigo_ldl_set_batch_mask(amigo_ldl* h, const int* mask, void* stream);
 
 /* Per-node pivot and delayed-pivot counts of the last factorization
  * (MatrixFactor::get_factor, include/sparse_ldl.h:411-445). */

@@ -1,0 +1,157 @@
+"""GPU tests of the batched form: many problems in one block-diagonal KKT.
+
+* Per-block inertia and solves of a union of independent test matrices equal
+  each matrix factored alone (bitwise: the union keeps every block's own
+  symbolic plan, and each front is computed by the same kernel code).
+* A masked factorization refactors only the blocks it names; the others
+  keep their factors (solves with them still give their old solutions).
+* The lockstep interior-point driver reproduces every problem's
+  single-problem solve bit for bit (iterations, inertia sequence, solution),
+  in FP64 and in the mixed-precision form.
+"""
+
+import numpy as np
+import pytest
+
+from tests import matrices as M
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _union(As, order="AMD", scaling=False):
+    from scipy.sparse import block_diag
+
+    from amigo_amd.sparse import CSRMat, SparseLDL
+
+    blocks = []
+    for A in As:
+        A = A.tocsr()
+        A.sort_indices()
+        blocks.append(SparseLDL(CSRMat.from_scipy(A, device="cpu"), order=order))
+    U = block_diag([A.tocsr() for A in As], format="csr")
+    U.sort_indices()
+    mat = CSRMat.from_scipy(U)
+    ldl = SparseLDL.union(mat, blocks, scaling=scaling)
+    offs = np.concatenate([[0], np.cumsum([A.shape[0] for A in As])])
+    K = [b.sizes()["num_nodes"] for b in blocks]
+    ldl.set_batch(np.concatenate([[0], np.cumsum(K)]).astype(np.int32))
+    return U, mat, ldl, offs
+
+
+def _alone(A, order="AMD", scaling=False, shift=None):
+    from amigo_amd.sparse import CSRMat, SparseLDL
+
+    mat = CSRMat.from_scipy(A)
+    ldl = SparseLDL(mat, order=order, scaling=scaling)
+    if shift is None:
+        ldl.factor()
+    else:
+        ldl.factor(shift=torch.full((A.shape[0],), shift, dtype=torch.float64, device="cuda"))
+    return ldl
+
+
+def _mats():
+    As = [M.small(i)[0] for i in range(3)]
+    As.append(M.grid_kkt(6, seed=3)[0])
+    As.append(M.kkt(40, 15, seed=2, indefinite_h=True))
+    As.append(M.poisson_2d(10, seed=0))
+    # Full diagonal patterns, so that a diagonal shift keeps the pattern.
+    return [M.with_full_diagonal(A.tocsr()) for A in As]
+
+
+@pytest.mark.parametrize("scaling", [False, True])
+def test_union_inertia_and_solves_equal_each_block_alone(scaling):
+    As = _mats()
+    U, mat, ldl, offs = _union(As, scaling=scaling)
+    ldl.factor()
+    npos, nneg, nzero, nnan = ldl.get_batch_inertia()
+    rng = np.random.default_rng(0)
+    b = rng.normal(size=U.shape[0])
+    xu = torch.tensor(b, device="cuda")
+    ldl.solve(xu)
+    xu = xu.cpu().numpy()
+    for k, A in enumerate(As):
+        a = _alone(A, scaling=scaling)
+        assert (npos[k], nneg[k]) == a.get_inertia()
+        assert nnan[k] == 0
+        xa = torch.tensor(b[offs[k]:offs[k + 1]], device="cuda")
+        a.solve(xa)
+        assert np.array_equal(xa.cpu().numpy(), xu[offs[k]:offs[k + 1]])
+
+
+def test_masked_factorization_keeps_left_out_factors():
+    As = _mats()
+    U, mat, ldl, offs = _union(As)
+    n = U.shape[0]
+    dev = "cuda"
+    s0 = torch.zeros(n, dtype=torch.float64, device=dev)
+    ldl.factor(shift=s0)
+    rng = np.random.default_rng(1)
+    b = rng.normal(size=n)
+    x0 = torch.tensor(b, device=dev)
+    ldl.solve(x0)
+    x0 = x0.cpu().numpy()
+    # Shift every block, refactor only blocks 0, 2, 4.
+    s1 = torch.full((n,), 0.5, dtype=torch.float64, device=dev)
+    mask = np.array([k % 2 == 0 for k in range(len(As))])
+    ldl.set_batch_mask(mask)
+    ldl.factor(shift=s1)
+    npos, nneg, _, _ = ldl.get_batch_inertia()
+    ldl.set_batch_mask(None)
+    x1 = torch.tensor(b, device=dev)
+    ldl.solve(x1)
+    x1 = x1.cpu().numpy()
+    for k, A in enumerate(As):
+        seg = slice(offs[k], offs[k + 1])
+        if mask[k]:
+            a = _alone(A, shift=0.5)
+            assert (npos[k], nneg[k]) == a.get_inertia()
+            xa = torch.tensor(b[seg], device=dev)
+            a.solve(xa)
+            assert np.array_equal(xa.cpu().numpy(), x1[seg])
+        else:
+            # Left out: still the factor of the unshifted matrix.
+            assert np.array_equal(x0[seg], x1[seg])
+
+
+def _problems():
+    from amigo_amd.problems import cart_pole, double_integrator, elliptic
+
+    ps = [cart_pole.seeded(200 + s, nmin=16, nmax=80) for s in range(5)]
+    ps.append(double_integrator.build(N=24))
+    ps.append(elliptic.EllipticControl(m=10, dim=2, seed=4))
+    return ps
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batched_solves_equal_single_problem_solves(precision):
+    from amigo_amd.batch import BatchOptimizer
+    from amigo_amd.optimizer import Optimizer
+
+    probs = _problems()
+    opts = {"record_objective": False}
+    bo = BatchOptimizer(probs, order="ND", scaling=True, precision=precision)
+    res = bo.optimize(opts)
+    xs = bo.get_solution()
+    assert all(r["converged"] for r in res)
+    for k, p in enumerate(probs):
+        o = Optimizer(p, order="ND", scaling=True, precision=precision)
+        d = o.optimize(opts)
+        assert d["converged"]
+        assert d["num_iterations"] == res[k]["num_iterations"], k
+        assert d["inertia_history"] == res[k]["inertia_history"], k
+        assert np.array_equal(o.get_solution(), xs[k]), (k, np.max(np.abs(o.get_solution() - xs[k])))
+
+
+def test_batched_solve_repeats_bitwise():
+    from amigo_amd.batch import BatchOptimizer
+
+    probs = _problems()[:4]
+    bo = BatchOptimizer(probs)
+    bo.optimize({"record_objective": False})
+    x1 = [x.copy() for x in bo.get_solution()]
+    bo.optimize({"record_objective": False})
+    x2 = bo.get_solution()
+    assert all(np.array_equal(a, b) for a, b in zip(x1, x2))
