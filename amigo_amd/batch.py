@@ -1048,6 +1048,10 @@ class BatchOptimizer:
         self.diag[be.pidx_g] = 1.0
         hs = be.apply_hessian_scaling if be.scaling_active else None
         self.solver.factor(0.0, self.x, self.diag, post_hessian=hs)
+        p, q, _, nan = self.solver.inertia()
+        # The solver-level inertia record (AmigoSolver.inertia_history)
+        # starts with this factorization.
+        self._ls_inertia = [(int(p[k]), int(q[k])) if nan[k] == 0 else None for k in range(self.K)]
         self.solver.solve(self.res, self.px)
         lam = self.px[be.cidx_g].abs()
         lmax = torch.zeros(self.K, dtype=torch.float64, device=self.x.device)
@@ -1520,7 +1524,11 @@ class BatchOptimizer:
         be.copy_x(self.px, self.wd_px, m)
 
     # ---- main loop (ipm_driver.py:106-483) ------------------------------------
-    def optimize(self, options={}):
+    def optimize(self, options={}, on_done=None):
+        """Solve every problem; returns one result dict per problem (the
+        keys of Optimizer.optimize, plus "batched").  on_done(k, result, x)
+        is called as soon as problem k has finished, with its unscaled
+        primal-dual solution on the host."""
         start = time.perf_counter()
         options = get_default_options(options)
         if options["filter_monotone_mode"] if "filter_monotone_mode" in options else False:
@@ -1595,6 +1603,14 @@ class BatchOptimizer:
                     data[k]["acceptable"] = True
                     if s == PRECISION_FLOOR:
                         data[k]["precision_floor"] = True
+            if on_done is not None:
+                for k in np.nonzero(A & (status != CONTINUE))[0]:
+                    data[k]["inertia_history"] = list(ic.history[k])
+                    data[k]["solver_inertia_history"] = self._solver_history(k)
+                    data[k]["num_iterations"] = len(data[k]["iterations"]) - 1
+                    data[k]["time"] = time.perf_counter() - start
+                    data[k]["batched"] = True
+                    on_done(int(k), data[k], self._solution_of(int(k)))
             active &= status == CONTINUE
             A = active.copy()
             if not A.any():
@@ -1678,10 +1694,19 @@ class BatchOptimizer:
         self.results = data
         for k in range(K):
             data[k]["inertia_history"] = list(ic.history[k])
+            data[k]["solver_inertia_history"] = self._solver_history(k)
             data[k]["num_iterations"] = len(data[k]["iterations"]) - 1
             data[k]["time"] = time.perf_counter() - start
             data[k]["batched"] = not self.fallback[k]
         self.batch_time = time.perf_counter() - start
+        if on_done is not None:
+            # Problems still iterating when the iteration limit ended the loop.
+            for k in np.nonzero(active)[0]:
+                data[k]["inertia_history"] = list(ic.history[k])
+                data[k]["solver_inertia_history"] = self._solver_history(int(k))
+                data[k]["num_iterations"] = len(data[k]["iterations"]) - 1
+                data[k]["batched"] = True
+                on_done(int(k), data[k], self._solution_of(int(k)))
         F = np.nonzero(self.fallback)[0]
         self._alone = {}
         if F.size:
@@ -1698,6 +1723,7 @@ class BatchOptimizer:
                     o.solver.ldl.set_graphs(False)
                     d = o.optimize(options)
                     stream.synchronize()
+                d["solver_inertia_history"] = list(o.solver.inertia_history)
                 return k, o, d
 
             with cf.ThreadPoolExecutor(max_workers=min(8, F.size)) as ex:
@@ -1705,8 +1731,28 @@ class BatchOptimizer:
                     d["batched"] = False
                     data[k] = d
                     self._alone[k] = o
+                    if on_done is not None:
+                        on_done(k, d, o.get_solution())
         self.total_time = time.perf_counter() - start
         return data
+
+    def _solver_history(self, k):
+        """Inertia of every factorization of problem k, as the
+        single-problem solver records it (AmigoSolver.inertia_history)."""
+        h = [] if getattr(self, "_ls_inertia", None) is None or self._ls_inertia[k] is None \
+            else [self._ls_inertia[k]]
+        return h + list(self.ic.history[k])
+
+    def _solution_of(self, k):
+        """Problem k's unscaled x (host copy) from the batch vectors."""
+        be = self.be
+        seg = slice(int(self.bp.xseg[k]), int(self.bp.xseg[k + 1]))
+        x = self.vars.x[seg].cpu().numpy().copy()
+        if be.scaling_active:
+            c0, c1 = int(be.cseg[k]), int(be.cseg[k + 1])
+            ci = be.cidx_g_h[c0:c1] - int(self.bp.xseg[k])
+            x[ci] /= be.constr_scale[c0:c1].cpu().numpy()
+        return x
 
     def get_solution(self, k=None):
         """Primal and multiplier values in the unscaled space (host copies),

@@ -258,6 +258,11 @@ class LdlNumeric {
   std::vector<int64_t> h_big_woff_;
   std::vector<int> h_nptr_, h_nvars_;
   int* d_row_blk_ = nullptr;        // [n] block of every row (batches)
+  // Per node: the LDS the front's workgroup would get with its own block
+  // alone (its level bucket's largest slot among that block's fronts), so
+  // that the choice between the LDS and the global-memory factorization
+  // does not depend on the other blocks of a batch.
+  int* d_node_cap_ = nullptr;
   bool masked() const { return nbatch_ > 0 && !mask_all_; }
   double flops_ = 0.0, solve_bytes_ = 0.0;
   bool factored_ = false;

@@ -608,7 +608,8 @@ __global__ __launch_bounds__(NT) void level_kernel(
     DevPlan P, DevState S, const double* __restrict__ vals,
     const double* __restrict__ shift, const int* __restrict__ lnodes,
     const int* __restrict__ big_mask, int lds_doubles, double ustab,
-    double pivot_tol, int chol, int dcap_big, const double* __restrict__ sc) {
+    double pivot_tol, int chol, int dcap_big, const double* __restrict__ sc,
+    const int* __restrict__ node_cap) {
   extern __shared__ double smem[];  // F (lds_doubles) then the cached column
   __shared__ VI red[2 * NT / 64];
   __shared__ VI cred[2 * NT / 64];
@@ -681,7 +682,8 @@ __global__ __launch_bounds__(NT) void level_kernel(
   // Fronts larger than the LDS limit go to the blocked path (decided on the
   // actual size, which includes delayed pivots).
   const bool big = m > LDS_MAX_M;
-  const bool in_lds = !big && (long long)m * m <= lds_doubles;
+  const long long cap = node_cap ? (long long)node_cap[node] : (long long)lds_doubles;
+  const bool in_lds = !big && (long long)m * m <= cap;
   double* F = in_lds ? smem : G;
   const int ld = m;
 
@@ -2192,7 +2194,8 @@ LdlNumeric::~LdlNumeric() {
   if (d_blk_layout_) (void)hipFree(d_blk_layout_);
   if (d_tops_batch_) (void)hipFree(d_tops_batch_);
   for (void* q : {(void*)d_m_level_nodes_, (void*)d_m_big_nodes_, (void*)d_m_big_root_,
-                  (void*)d_m_big_ns_, (void*)d_m_big_woff_, (void*)d_row_blk_})
+                  (void*)d_m_big_ns_, (void*)d_m_big_woff_, (void*)d_row_blk_,
+                  (void*)d_node_cap_})
     if (q) (void)hipFree(q);
 }
 
@@ -2489,7 +2492,11 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   }
   base_rcap_ = (unsigned long long)(std::max(1.0, delay_growth) * rneed) + 4096;
   base_icap_ = (unsigned long long)(std::max(1.0, delay_growth) * ineed) + 4096;
-  return ensure_pools(base_rcap_, base_icap_, err);
+  const int rc = ensure_pools(base_rcap_, base_icap_, err);
+  // The plan was uploaded through the null stream: make it visible to
+  // kernels on any (non-blocking) stream before the first factorization.
+  HIPCHK(hipDeviceSynchronize());
+  return rc;
 }
 
 int LdlNumeric::ensure_pools(unsigned long long rcap, unsigned long long icap,
@@ -2576,7 +2583,8 @@ int LdlNumeric::enqueue_factor(const double* vals, const double* shift,
                          shift, LN + pt[0], d_big_mask_, lds,
                          prm.ustab, prm.pivot_tol, cholesky ? 1 : 0,
                          sync_mode ? -1 : DCAP_BIG,
-                         (const double*)(scaling_ ? scale_ : nullptr));
+                         (const double*)(scaling_ ? scale_ : nullptr),
+                         (const int*)(nbatch_ ? d_node_cap_ : nullptr));
     }
     tk_end(stream, TK_LEVEL);
     HIPCHK(hipGetLastError());
@@ -3031,6 +3039,33 @@ int LdlNumeric::set_batch(int nblocks, const int* node_seg, std::string& err) {
     HIPCHK(hipMalloc(&d_row_blk_, sizeof(int) * std::max(n_, 1)));
     HIPCHK(hipMemcpy(d_row_blk_, rb.data(), sizeof(int) * n_, hipMemcpyHostToDevice));
   }
+  {
+    // LDS slot each front gets with its own block alone.
+    std::vector<int> cap(K_, 0);
+    const int nlev = num_levels();
+    auto side = [&](int k) { return std::min(h_ns_[k] + h_cs_[k] + 32, LDS_MAX_M); };
+    std::vector<int> mx(nblocks, 0);
+    for (int l = 0; l < nlev; l++) {
+      const int parts[2][2] = {{level_ptr_[l], level_split_[l]},
+                               {level_split_[l], level_ptr_[l + 1]}};
+      for (const auto& pt : parts) {
+        for (int t = pt[0]; t < pt[1]; t++) mx[h_node_blk_[level_nodes_[t]]] = 0;
+        for (int t = pt[0]; t < pt[1]; t++) {
+          const int k = level_nodes_[t];
+          int& v = mx[h_node_blk_[k]];
+          v = std::max(v, side(k));
+        }
+        for (int t = pt[0]; t < pt[1]; t++) {
+          const int k = level_nodes_[t];
+          const int v = mx[h_node_blk_[k]];
+          cap[k] = v * v;
+        }
+      }
+    }
+    fr(d_node_cap_);
+    HIPCHK(hipMalloc(&d_node_cap_, sizeof(int) * std::max(K_, 1)));
+    HIPCHK(hipMemcpy(d_node_cap_, cap.data(), sizeof(int) * K_, hipMemcpyHostToDevice));
+  }
   fr(d_m_level_nodes_);
   fr(d_m_big_nodes_);
   fr(d_m_big_root_);
@@ -3087,6 +3122,9 @@ int LdlNumeric::layout_batch(std::string& err) {
   // the re-laid-out pools hold no factor.
   drop_graphs();
   factored_ = false;
+  // Tables written through the null stream must be complete before kernels
+  // on the caller's (possibly non-blocking) stream read them.
+  HIPCHK(hipDeviceSynchronize());
   return 0;
 }
 

@@ -165,6 +165,177 @@ class SolveServer:
                     fut.set_exception(e)
 
 
+class BatchSolveServer:
+    """Continuous batching with fused launches: requests queued while earlier
+    batches run are gathered into the next lockstep batch
+    (batch.BatchOptimizer), whose problems share every level-kernel launch
+    of the KKT factorization and every IPM kernel launch.  Up to
+    `max_batches` batches run at once, each on its own HIP stream, so a new
+    batch starts while an older one finishes its slowest problems.  A batch
+    takes at most `max_batch` requests and is admitted by predicted device
+    memory (the sum of its problems' SparseLDL.memory_bytes and vectors)
+    against the budget.  Each request's result is returned when its own
+    problem finishes, not when its batch does, and equals the request
+    solved alone bit for bit.
+
+        with BatchSolveServer(max_batch=512) as srv:
+            fut = srv.submit({"type": "cart_pole", "N": 256, "seed": 7})
+            res = fut.result()
+    """
+
+    def __init__(self, max_batch=512, max_batches=2, gather_s=0.005, memory_budget=None,
+                 precision="mixed", options=None, device="cuda", scaling=True, order="ND"):
+        self.device = torch.device(device)
+        if memory_budget is None:
+            free, _ = torch.cuda.mem_get_info(self.device)
+            memory_budget = int(0.8 * free)
+        self.budget = int(memory_budget)
+        self.max_batch = int(max_batch)
+        self.gather_s = float(gather_s)
+        self.precision = precision
+        self.options = dict(options or {})
+        self.scaling = scaling
+        self.order = order
+        self._q = queue.Queue()
+        self._cv = threading.Condition()
+        self._inflight = 0
+        self._inflight_bytes = 0
+        self._free_slots = int(max_batches)
+        self.peak_inflight = 0
+        self.peak_bytes = 0
+        self.batches = []  # (size, wall seconds) of every batch
+        self.batch_specs = []
+        self._closed = False
+        self._workers = []
+        self._disp = threading.Thread(target=self._dispatch, daemon=True, name="amigo-batcher")
+        self._disp.start()
+
+    def submit(self, spec, arrival=None):
+        fut = cf.Future()
+        self._q.put((spec, fut, time.perf_counter() if arrival is None else arrival))
+        return fut
+
+    def close(self):
+        if self._closed:
+            return
+        self._closed = True
+        self._q.put(None)
+        self._disp.join()
+        for t in list(self._workers):
+            t.join()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- dispatcher: gather queued requests into batches --------------------
+    def _dispatch(self):
+        pending = []
+        stop = False
+        while not stop or pending:
+            if not pending:
+                item = self._q.get()
+                if item is None:
+                    stop = True
+                    continue
+                pending.append(item)
+            # Gather what arrives within gather_s (or until the batch is full).
+            t_end = time.perf_counter() + self.gather_s
+            while len(pending) < self.max_batch:
+                try:
+                    item = self._q.get(timeout=max(0.0, t_end - time.perf_counter()))
+                except queue.Empty:
+                    break
+                if item is None:
+                    stop = True
+                    break
+                pending.append(item)
+            with self._cv:
+                while self._free_slots == 0:
+                    self._cv.wait()
+                self._free_slots -= 1
+            # Requests that arrived while waiting for a slot join this batch.
+            while len(pending) < self.max_batch:
+                try:
+                    item = self._q.get_nowait()
+                except queue.Empty:
+                    break
+                if item is None:
+                    stop = True
+                    break
+                pending.append(item)
+            batch, pending = pending[: self.max_batch], pending[self.max_batch:]
+            t = threading.Thread(target=self._run_batch, args=(batch,), daemon=True)
+            self._workers = [w for w in self._workers if w.is_alive()] + [t]
+            t.start()
+
+    # ---- one batch -----------------------------------------------------------
+    def _run_batch(self, batch):
+        from .batch import BatchOptimizer, problem_ldl
+
+        stream = torch.cuda.Stream(self.device)
+        need = 0
+        admitted = False
+        try:
+            with torch.cuda.device(self.device), torch.cuda.stream(stream):
+                t_start = time.perf_counter()
+                probs = [build_problem(spec, self.device) for spec, _, _ in batch]
+                need = sum(problem_ldl(p, self.order).memory_bytes() + 8 * (p.nnz + 24 * p.n)
+                           for p in probs)
+                self._admit(len(batch), need)
+                admitted = True
+                t_admit = time.perf_counter()
+                bo = BatchOptimizer(probs, order=self.order, scaling=self.scaling,
+                                    precision=self.precision)
+                # No HIP-graph capture beside other batches' streams.
+                bo.solver.ldl.set_graphs(False)
+
+                def done(k, out, x):
+                    spec, fut, t_arr = batch[k]
+                    t_done = time.perf_counter()
+                    fut.set_result({
+                        "spec": spec,
+                        "x": x,
+                        "converged": bool(out["converged"]),
+                        "iterations": int(out["num_iterations"]),
+                        "nlp_error": float(out["iterations"][-1]["nlp_error"]),
+                        "inertia": list(out.get("solver_inertia_history", out["inertia_history"])),
+                        "batched": bool(out.get("batched", True)),
+                        "batch_size": len(batch),
+                        "latency_s": t_done - t_arr,
+                        "queue_s": t_admit - t_arr,
+                        "build_s": t_admit - t_start,
+                        "solve_s": t_done - t_admit,
+                    })
+
+                bo.optimize(dict(self.options), on_done=done)
+                stream.synchronize()
+                self.batches.append((len(batch), time.perf_counter() - t_start))
+                self.batch_specs.append([spec for spec, _, _ in batch])
+        except Exception as e:  # every unfinished request gets the error
+            for _, fut, _ in batch:
+                if not fut.done():
+                    fut.set_exception(e)
+        finally:
+            with self._cv:
+                if admitted:
+                    self._inflight -= len(batch)
+                    self._inflight_bytes -= need
+                self._free_slots += 1
+                self._cv.notify_all()
+
+    def _admit(self, count, need):
+        with self._cv:
+            while self._inflight > 0 and self._inflight_bytes + need > self.budget:
+                self._cv.wait()
+            self._inflight += count
+            self._inflight_bytes += need
+            self.peak_inflight = max(self.peak_inflight, self._inflight)
+            self.peak_bytes = max(self.peak_bytes, self._inflight_bytes)
+
+
 def request_mix(n, seed=0, cart_pole_frac=0.7, cart_n=(32, 1024), side=(32, 192)):
     """Seeded request specs: cart-pole swing-ups with N ~ U{cart_n} and 2-D
     elliptic control problems with side ~ U{side}."""

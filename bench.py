@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the other-precision and serving measurements")
+    ap.add_argument("--forms", default="precision,serving,batching_slots,batching,serving_lockstep",
+                    help="comma-separated forms to measure")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="MiB of one device copy run first (counter calibration)")
     ap.add_argument("--serve-requests", type=int, default=32)
@@ -59,6 +61,9 @@ def parse():
                     help="cart-pole problems solved in one lockstep batch")
     ap.add_argument("--lockstep-sample", type=int, default=16,
                     help="of those, solved again one at a time (rate and identity check)")
+    ap.add_argument("--serve-lockstep-requests", type=int, default=96,
+                    help="requests per arrival rate of the lockstep-batching server")
+    ap.add_argument("--serve-lockstep-rates", default="100,1000,5000")
     ap.add_argument("--timing-steps", type=int, default=1,
                     help="solves timed per kernel class with HIP events (roofline)")
     return ap.parse_args()
@@ -215,17 +220,31 @@ def _inertia_of_last_solve(opt, before):
 
 
 def forms(args, prob, opt, outs, lat):
-    """The mixed-precision form on the same workload, and the served form on
-    seeded request mixes (each compared with the same requests solved one
-    at a time in FP64)."""
+    """The other precision of the same workload, and the served and batched
+    forms on seeded request mixes, each against the same requests solved one
+    at a time in FP64."""
+    res = {}
+    want = set(f for f in args.forms.split(",") if f)
+    if "precision" in want:
+        res["precision"] = precision_form(args, prob, opt)
+    if "serving" in want:
+        res["serving"] = serving_slots(args)
+    if "batching_slots" in want:
+        res["batching_slots"] = batching_slots(args)
+    if "batching" in want and args.lockstep_problems > 0:
+        res["batching"] = lockstep_batching(args)
+    if "serving_lockstep" in want and args.serve_lockstep_requests > 0:
+        res["serving_lockstep"] = lockstep_serving(args)
+    return res
+
+
+def precision_form(args, prob, opt):
+    """The other precision of the single large solve."""
     import numpy as np
     import torch
 
     from amigo_amd.optimizer import Optimizer
-    from amigo_amd.serve import SolveServer, request_mix, run_poisson
 
-    res = {}
-    # Precision: the other form of the single large solve.
     other = "mixed" if args.precision == "fp64" else "fp64"
     h0 = len(opt.solver.inertia_history)
     opt.optimize({"record_objective": False})
@@ -242,7 +261,7 @@ def forms(args, prob, opt, outs, lat):
         torch.cuda.synchronize()
         lat2.append(time.perf_counter() - t)
     x2 = o2.get_solution()
-    res["precision"] = {
+    out = {
         "form": other,
         "value": len(lat2) / sum(lat2),
         "p50_ms": 1e3 * float(np.percentile(lat2, 50)),
@@ -253,12 +272,19 @@ def forms(args, prob, opt, outs, lat):
         "max_abs_diff_vs_" + args.precision: float(np.max(np.abs(x2 - x_ref))),
         "fp64_fallbacks": o2.solver.fp64_fallbacks,
     }
-    print(f"[bench] {other}: {res['precision']['value']:.3f} solves/s", file=sys.stderr, flush=True)
+    print(f"[bench] {other}: {out['value']:.3f} solves/s", file=sys.stderr, flush=True)
     del o2
+    return out
 
-    # Serving: an open-loop Poisson stream of cart-pole and 2-D elliptic
-    # requests, several in flight (continuous batching), against the same
-    # requests solved one at a time.
+
+def serving_slots(args):
+    """Slot serving: an open-loop Poisson stream of cart-pole and 2-D
+    elliptic requests, several in flight on their own streams, against the
+    same requests solved one at a time."""
+    import numpy as np
+
+    from amigo_amd.serve import SolveServer, request_mix, run_poisson
+
     specs = request_mix(args.serve_requests, seed=args.seed + 1)
     with SolveServer(slots=1) as srv:
         t = time.perf_counter()
@@ -267,7 +293,7 @@ def forms(args, prob, opt, outs, lat):
     with SolveServer(slots=args.serve_slots) as srv:
         st = run_poisson(srv, specs, rate=args.serve_rate, seed=args.seed + 2)
     same = all(np.array_equal(a["x"], b["x"]) for a, b in zip(seq, st["results"]))
-    res["serving"] = {
+    out = {
         "requests": len(specs),
         "mix": "70% cart-pole N~U{32..1024}, 30% 2-D elliptic side~U{32..192}",
         "arrival_rate_per_s": args.serve_rate,
@@ -284,9 +310,16 @@ def forms(args, prob, opt, outs, lat):
     }
     print(f"[bench] serving: {st['solves_per_s']:.2f} solves/s (sequential {seq_rate:.2f})",
           file=sys.stderr, flush=True)
+    return out
 
-    # Batching: independent cart-pole swing-ups with N ~ U{16..512} and
-    # seeded masses and length, all queued at once, several in flight.
+
+def batching_slots(args):
+    """Independent cart-pole swing-ups with N ~ U{16..512} and seeded masses
+    and length, all queued at once, several in flight on slot streams."""
+    import numpy as np
+
+    from amigo_amd.serve import SolveServer, request_mix
+
     cp = request_mix(args.batch_problems, seed=args.seed + 3, cart_pole_frac=1.0, cart_n=(16, 512))
     with SolveServer(slots=1) as srv:
         t = time.perf_counter()
@@ -296,7 +329,7 @@ def forms(args, prob, opt, outs, lat):
         t = time.perf_counter()
         bat = [f.result() for f in [srv.submit(sp) for sp in cp]]
         bat_rate = len(bat) / (time.perf_counter() - t)
-    res["batching_slots"] = {
+    out = {
         "problems": len(cp),
         "workload": "cart-pole swing-up, N~U{16..512}, T=2 s, mc~U(0.5,2), mp~U(0.1,0.5), l~U(0.3,1)",
         "slots": args.batch_slots,
@@ -309,9 +342,60 @@ def forms(args, prob, opt, outs, lat):
     }
     print(f"[bench] batching (slots): {bat_rate:.2f} solves/s (sequential {seq_rate:.2f})", file=sys.stderr,
           flush=True)
-    if args.lockstep_problems > 0:
-        res["batching"] = lockstep_batching(args)
-    return res
+    return out
+
+
+def lockstep_serving(args):
+    """The served form with fused launches: open-loop Poisson streams of the
+    request mix (70% cart-pole swing-ups N ~ U{32..1024}, 30% 2-D elliptic
+    problems side ~ U{32..192}) at each rate, gathered into lockstep batches
+    as they arrive (BatchSolveServer, mixed precision), against the first
+    requests of the same mix solved one at a time in plain FP64."""
+    import numpy as np
+
+    from amigo_amd.serve import BatchSolveServer, SolveServer, request_mix, run_poisson
+
+    n = args.serve_lockstep_requests
+    specs = request_mix(n, seed=args.seed + 11)
+    ns = min(16, n)
+    with SolveServer(slots=1, precision="fp64") as srv:
+        t = time.perf_counter()
+        seq = [f.result() for f in [srv.submit(sp) for sp in specs[:ns]]]
+        seq_rate = ns / (time.perf_counter() - t)
+    # The same requests one at a time in the served precision: the bitwise
+    # reference for the batched results.
+    with SolveServer(slots=1, precision="mixed") as srv:
+        seq_m = [f.result() for f in [srv.submit(sp) for sp in specs[:ns]]]
+    out = {"requests_per_rate": n,
+           "mix": "70% cart-pole N~U{32..1024}, 30% 2-D elliptic side~U{32..192}",
+           "form": "lockstep batches of the requests queued while earlier batches run, up to 4 batches "
+                   "at once on their own streams, mixed precision; each result returned when its own "
+                   "problem finishes",
+           "sequential_fp64_sample": ns, "sequential_fp64_solves_per_s": seq_rate, "rates": []}
+    same = True
+    for rate in [float(r) for r in args.serve_lockstep_rates.split(",") if r]:
+        with BatchSolveServer(max_batch=512, max_batches=4) as srv:
+            st = run_poisson(srv, specs, rate=rate, seed=args.seed + 12)
+        sizes = [b for b, _ in srv.batches]
+        same &= all(np.array_equal(a["x"], b["x"]) for a, b in zip(seq_m, st["results"][:ns]))
+        dmax = max(float(np.max(np.abs(a["x"] - b["x"]))) for a, b in zip(seq, st["results"][:ns]))
+        out["rates"].append({
+            "arrival_rate_per_s": rate,
+            "value": st["solves_per_s"],
+            "unit": "solves/s",
+            "p50_ms": st["p50_ms"],
+            "p99_ms": st["p99_ms"],
+            "converged": st["converged"],
+            "batches": len(sizes),
+            "mean_batch_size": float(np.mean(sizes)) if sizes else 0.0,
+            "speedup_vs_sequential_fp64": st["solves_per_s"] / seq_rate,
+            "sample_max_abs_diff_vs_fp64": dmax,
+        })
+        print(f"[bench] lockstep serving at {rate:.0f} req/s: {st['solves_per_s']:.1f} solves/s, "
+              f"p50 {st['p50_ms']:.0f} ms, p99 {st['p99_ms']:.0f} ms, {len(sizes)} batches",
+              file=sys.stderr, flush=True)
+    out["sample_identical_to_sequential_same_precision"] = bool(same)
+    return out
 
 
 def lockstep_batching(args):

@@ -64,3 +64,32 @@ def test_poisson_stream_reports_latency():
     assert st["converged"] == len(specs)
     assert 0 < st["p50_ms"] <= st["p99_ms"]
     assert st["solves_per_s"] > 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lockstep_batches_equal_requests_alone(precision):
+    """Requests gathered into lockstep batches (fused level-kernel launches)
+    give bitwise the result each gets alone."""
+    from amigo_amd.serve import BatchSolveServer
+
+    alone, _ = _run(1, SPECS, precision=precision)
+    with BatchSolveServer(max_batch=4, max_batches=2, gather_s=0.05, precision=precision) as srv:
+        futs = [srv.submit(s) for s in SPECS]
+        res = [f.result() for f in futs]
+    sizes = [b for b, _ in srv.batches]
+    assert sum(sizes) == len(SPECS) and max(sizes) > 1
+    for a, b in zip(alone, res):
+        assert b["converged"]
+        assert a["iterations"] == b["iterations"]
+        assert a["inertia"] == b["inertia"]
+        assert np.array_equal(a["x"], b["x"])
+
+
+def test_lockstep_poisson_stream():
+    from amigo_amd.serve import BatchSolveServer, request_mix, run_poisson
+
+    specs = request_mix(12, seed=5, cart_n=(32, 128), side=(32, 48))
+    with BatchSolveServer(max_batch=8, max_batches=2) as srv:
+        st = run_poisson(srv, specs, rate=200.0, seed=6)
+    assert st["converged"] == len(specs)
+    assert 0 < st["p50_ms"] <= st["p99_ms"]
