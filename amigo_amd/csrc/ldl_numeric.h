@@ -235,6 +235,13 @@ class LdlNumeric {
   int* d_batch_seg_ = nullptr;      // [nbatch+1]
   int* d_batch_inert_ = nullptr;    // [nbatch*4]
   std::vector<int> binert_;
+  // Pinned host buffers for the device-to-host results and the mask
+  // uploads (asynchronous copies whose host side outlives them).
+  int* h_flags_pin_ = nullptr;      // [8]
+  int* h_binert_pin_ = nullptr;     // [4 * nbatch]
+  char* h_stage_pin_ = nullptr;     // mask / compacted-list upload staging
+  size_t h_stage_cap_ = 0;
+  int stage(const void* src, size_t bytes, size_t& off, const void** out, std::string& err);
   std::vector<int> h_node_blk_, h_blk_active_;
   std::vector<unsigned long long> blk_rcap_h_, blk_icap_h_;
   int* d_node_blk_ = nullptr;

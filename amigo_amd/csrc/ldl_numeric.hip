@@ -14,6 +14,8 @@
 // root (sparse_ldl.h:1208-1230).  Inertia follows sparse_ldl.h:1917-1969.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -87,10 +89,16 @@ __device__ __forceinline__ void swapd(double& a, double& b) {
 }
 
 // Symmetric interchange of rows/columns a < b of the lower triangle
-// (sparse_ldl.h:1252-1274).  Ends with a barrier.
+// (sparse_ldl.h:1252-1274).  Starts and ends with a barrier: the callers'
+// pivot decisions read entries of F that the interchange moves (F[k, k],
+// F[imax, imax], F[imax, k]), and every wave must have read them before any
+// wave swaps.  Without the leading barrier a wave that falls behind (other
+// kernels sharing its CU) can read a swapped value and take a different
+// pivot step from the rest of its workgroup.
 template <int NTH>
 __device__ void symm_swap_t(double* F, int ld, int m, int* vars, int a, int b) {
   const int t0 = threadIdx.x;
+  __syncthreads();
   for (int t = t0; t < a; t += NTH) swapd(F[a + (size_t)t * ld], F[b + (size_t)t * ld]);
   if (t0 == 0) {
     swapd(F[a + (size_t)a * ld], F[b + (size_t)b * ld]);
@@ -2193,6 +2201,9 @@ LdlNumeric::~LdlNumeric() {
   if (d_blk_active_) (void)hipFree(d_blk_active_);
   if (d_blk_layout_) (void)hipFree(d_blk_layout_);
   if (d_tops_batch_) (void)hipFree(d_tops_batch_);
+  if (h_flags_pin_) (void)hipHostFree(h_flags_pin_);
+  if (h_binert_pin_) (void)hipHostFree(h_binert_pin_);
+  if (h_stage_pin_) (void)hipHostFree(h_stage_pin_);
   for (void* q : {(void*)d_m_level_nodes_, (void*)d_m_big_nodes_, (void*)d_m_big_root_,
                   (void*)d_m_big_ns_, (void*)d_m_big_woff_, (void*)d_row_blk_,
                   (void*)d_node_cap_})
@@ -2720,9 +2731,11 @@ int LdlNumeric::enqueue_factor(const double* vals, const double* shift,
 int LdlNumeric::finish_factor(hipStream_t stream, std::string& err) {
   HIPCHK(hipStreamSynchronize(stream));
   if (timing_) tk_collect();
+  if (!h_flags_pin_) HIPCHK(hipHostMalloc((void**)&h_flags_pin_, 8 * sizeof(int)));
   int flags[8];
-  HIPCHK(hipMemcpyAsync(flags, S_.flags, sizeof(flags), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(h_flags_pin_, S_.flags, sizeof(flags), hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));
+  for (int q = 0; q < 8; q++) flags[q] = h_flags_pin_[q];
   if (flags[0] != 0) return flags[0];
   if (nbatch_ > 0) {
     // Per-block signs; a non-finite pivot fails only its own block.
@@ -2730,9 +2743,10 @@ int LdlNumeric::finish_factor(hipStream_t stream, std::string& err) {
                        nbatch_, (const int*)d_batch_seg_, (const int*)S_.ninert, d_batch_inert_);
     HIPCHK(hipGetLastError());
     binert_.resize(4 * (size_t)nbatch_);
-    HIPCHK(hipMemcpyAsync(binert_.data(), d_batch_inert_, 4 * sizeof(int) * nbatch_,
+    HIPCHK(hipMemcpyAsync(h_binert_pin_, d_batch_inert_, 4 * sizeof(int) * nbatch_,
                           hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
+    std::copy(h_binert_pin_, h_binert_pin_ + 4 * (size_t)nbatch_, binert_.begin());
   } else if (flags[4] != 0) {
     err = "non-finite pivot (the matrix holds NaN or Inf)";
     return -6;
@@ -2787,11 +2801,8 @@ int LdlNumeric::factor(const double* vals, const double* shift, double ustab,
       }
       int r2 = layout_batch(err);
       if (r2) return r2;
-      std::fill(h_blk_active_.begin(), h_blk_active_.end(), 1);
-      mask_all_ = true;
-      HIPCHK(hipMemcpyAsync(d_blk_active_, h_blk_active_.data(), sizeof(int) * nbatch_,
-                            hipMemcpyHostToDevice, stream));
-      HIPCHK(hipStreamSynchronize(stream));
+      r2 = set_batch_mask(nullptr, stream, err);
+      if (r2) return r2;
       continue;
     }
     if (rc == -4) {
@@ -3013,6 +3024,9 @@ int LdlNumeric::set_batch(int nblocks, const int* node_seg, std::string& err) {
   HIPCHK(hipMemcpy(d_batch_seg_, node_seg, sizeof(int) * (nblocks + 1), hipMemcpyHostToDevice));
   nbatch_ = nblocks;
   binert_.assign(4 * (size_t)nblocks, 0);
+  if (h_binert_pin_) (void)hipHostFree(h_binert_pin_);
+  h_binert_pin_ = nullptr;
+  HIPCHK(hipHostMalloc((void**)&h_binert_pin_, 4 * sizeof(int) * (size_t)nblocks));
   // Node -> block, and each block's pool needs at delay_growth x its
   // static front sizes.
   h_node_blk_.assign(K_, 0);
@@ -3139,8 +3153,24 @@ int LdlNumeric::set_batch_mask(const int* mask, hipStream_t stream, std::string&
     on += h_blk_active_[b];
   }
   mask_all_ = on == nbatch_;
-  HIPCHK(hipMemcpyAsync(d_blk_active_, h_blk_active_.data(), sizeof(int) * nbatch_,
-                        hipMemcpyHostToDevice, stream));
+  // The uploads go through one pinned staging buffer, so that each copy is
+  // asynchronous and its source outlives it (drained at the end).
+  {
+    size_t need = sizeof(int) * nbatch_ + 64;
+    if (!mask_all_)
+      need += sizeof(int) * (size_t)K_ + (3 * sizeof(int) + sizeof(int64_t)) * big_nodes_.size() + 256;
+    if (need > h_stage_cap_) {
+      if (h_stage_pin_) (void)hipHostFree(h_stage_pin_);
+      h_stage_pin_ = nullptr;
+      h_stage_cap_ = 0;
+      HIPCHK(hipHostMalloc((void**)&h_stage_pin_, need));
+      h_stage_cap_ = need;
+    }
+  }
+  size_t soff = 0;
+  const void* src = nullptr;
+  if (stage(h_blk_active_.data(), sizeof(int) * nbatch_, soff, &src, err)) return -2;
+  HIPCHK(hipMemcpyAsync(d_blk_active_, src, sizeof(int) * nbatch_, hipMemcpyHostToDevice, stream));
   if (!mask_all_) {
     // Compacted launch lists (same order, same LDS buckets).
     const int nlev = num_levels();
@@ -3169,22 +3199,43 @@ int LdlNumeric::set_batch_mask(const int* mask, hipStream_t stream, std::string&
       }
       m_big_ptr_[l + 1] = int(m_big_nodes_.size());
     }
-    if (!m_level_nodes_.empty())
-      HIPCHK(hipMemcpyAsync(d_m_level_nodes_, m_level_nodes_.data(),
-                            sizeof(int) * m_level_nodes_.size(), hipMemcpyHostToDevice, stream));
+    auto up = [&](void* dst, const void* h, size_t bytes) -> int {
+      const void* sp = nullptr;
+      if (stage(h, bytes, soff, &sp, err)) return -2;
+      if (hipMemcpyAsync(dst, sp, bytes, hipMemcpyHostToDevice, stream) != hipSuccess) {
+        err = "hipMemcpyAsync";
+        return -2;
+      }
+      return 0;
+    };
+    if (!m_level_nodes_.empty() &&
+        up(d_m_level_nodes_, m_level_nodes_.data(), sizeof(int) * m_level_nodes_.size()))
+      return -2;
     if (!m_big_nodes_.empty()) {
       const size_t nb = m_big_nodes_.size();
-      HIPCHK(hipMemcpyAsync(d_m_big_nodes_, m_big_nodes_.data(), sizeof(int) * nb,
-                            hipMemcpyHostToDevice, stream));
-      HIPCHK(hipMemcpyAsync(d_m_big_root_, m_big_root_.data(), sizeof(int) * nb,
-                            hipMemcpyHostToDevice, stream));
-      HIPCHK(hipMemcpyAsync(d_m_big_ns_, m_big_ns_.data(), sizeof(int) * nb,
-                            hipMemcpyHostToDevice, stream));
-      HIPCHK(hipMemcpyAsync(d_m_big_woff_, m_big_woff_.data(), sizeof(int64_t) * nb,
-                            hipMemcpyHostToDevice, stream));
+      if (up(d_m_big_nodes_, m_big_nodes_.data(), sizeof(int) * nb) ||
+          up(d_m_big_root_, m_big_root_.data(), sizeof(int) * nb) ||
+          up(d_m_big_ns_, m_big_ns_.data(), sizeof(int) * nb) ||
+          up(d_m_big_woff_, m_big_woff_.data(), sizeof(int64_t) * nb))
+        return -2;
     }
   }
   HIPCHK(hipStreamSynchronize(stream));
+  return 0;
+}
+
+// Copy `bytes` of host data into the pinned staging buffer at `off` (8-byte
+// aligned); *out is the staged address.
+int LdlNumeric::stage(const void* src, size_t bytes, size_t& off, const void** out,
+                      std::string& err) {
+  off = (off + 7) & ~size_t(7);
+  if (off + bytes > h_stage_cap_) {
+    err = "staging buffer too small";
+    return -2;
+  }
+  std::memcpy(h_stage_pin_ + off, src, bytes);
+  *out = h_stage_pin_ + off;
+  off += bytes;
   return 0;
 }
 

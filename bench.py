@@ -377,7 +377,10 @@ def lockstep_serving(args):
         with BatchSolveServer(max_batch=512, max_batches=4) as srv:
             st = run_poisson(srv, specs, rate=rate, seed=args.seed + 12)
         sizes = [b for b, _ in srv.batches]
-        same &= all(np.array_equal(a["x"], b["x"]) for a, b in zip(seq_m, st["results"][:ns]))
+        mism = [(i, specs[i]["type"], bool(st["results"][i]["batched"]),
+                 float(np.max(np.abs(a["x"] - b["x"]))), a["iterations"], b["iterations"])
+                for i, (a, b) in enumerate(zip(seq_m, st["results"][:ns])) if not np.array_equal(a["x"], b["x"])]
+        same &= not mism
         dmax = max(float(np.max(np.abs(a["x"] - b["x"]))) for a, b in zip(seq, st["results"][:ns]))
         out["rates"].append({
             "arrival_rate_per_s": rate,
@@ -390,6 +393,7 @@ def lockstep_serving(args):
             "mean_batch_size": float(np.mean(sizes)) if sizes else 0.0,
             "speedup_vs_sequential_fp64": st["solves_per_s"] / seq_rate,
             "sample_max_abs_diff_vs_fp64": dmax,
+            "sample_mismatches_vs_same_precision": mism,
         })
         print(f"[bench] lockstep serving at {rate:.0f} req/s: {st['solves_per_s']:.1f} solves/s, "
               f"p50 {st['p50_ms']:.0f} ms, p99 {st['p99_ms']:.0f} ms, {len(sizes)} batches",

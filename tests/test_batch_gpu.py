@@ -155,3 +155,55 @@ def test_batched_solve_repeats_bitwise():
     bo.optimize({"record_objective": False})
     x2 = bo.get_solution()
     assert all(np.array_equal(a, b) for a, b in zip(x1, x2))
+
+
+def test_concurrent_factorizations_on_separate_streams_repeat_bitwise():
+    """Three batches factored at once from three threads, each on its own
+    stream, so that their kernels share the CUs: every factorization and
+    solve equals the one computed with the device to itself.  (Waves of a
+    workgroup that fall behind under such sharing must still see the pivot
+    entries of the front before any wave interchanges rows.)"""
+    import threading
+
+    from amigo_amd.batch import BatchOptimizer
+    from amigo_amd.problems import cart_pole, elliptic
+
+    def build():
+        ps = [cart_pole.seeded(300 + s, nmin=300, nmax=900) for s in range(6)]
+        ps.append(elliptic.EllipticControl(m=60, dim=2, seed=5))
+        return ps
+
+    def run(out, tag, reps):
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            bo = BatchOptimizer(build(), order="ND", scaling=True)
+            sol = bo.solver
+            sol.ldl.set_graphs(False)
+            bo.optimize({"max_iterations": 3, "record_objective": False})
+            sol.assemble_hessian(True, bo.x)
+            bo.be.apply_hessian_scaling(sol.hess)
+            diag = bo.diag.clone()
+            b = torch.tensor(np.random.default_rng(0).normal(size=diag.numel()), device="cuda")
+            xs = []
+            for _ in range(reps):
+                sol.add_diagonal_and_factor(diag)
+                x = b.clone()
+                sol.ldl.solve(x)
+                xs.append((x.cpu().numpy(), [tuple(v) for v in zip(*sol.ldl.get_batch_inertia())]))
+            s.synchronize()
+        out[tag] = xs
+
+    alone = {}
+    run(alone, "a", 1)
+    x0, in0 = alone["a"][0]
+    out = {}
+    ths = [threading.Thread(target=run, args=(out, t, 12)) for t in range(3)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    assert len(out) == 3
+    for t, xs in out.items():
+        for r, (x, inert) in enumerate(xs):
+            assert inert == in0, (t, r)
+            assert np.array_equal(x, x0), (t, r, float(np.max(np.abs(x - x0))))
