@@ -243,6 +243,7 @@ _SIGS.update(
         ),
         "amigo_ipm_batch_ir_update": (_I, [_PB, _PP, _PS, _V, _V, _V, _V, _V, _V, _V]),
         "amigo_ipm_batch_copy_state": (_I, [_PB, _PS, _PS, _I, _V]),
+        "amigo_segment_absmax": (_I, [_V, _V, _I, _V, _c_dbl_p, _V]),
     }
 )
 

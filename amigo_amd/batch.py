@@ -1073,9 +1073,16 @@ class BatchOptimizer:
         return s_d, s_c
 
     def _x_max(self):
-        m = torch.zeros(self.K, dtype=torch.float64, device=self.x.device)
-        m.scatter_reduce_(0, self.be.xprob, self.x.abs(), reduce="amax", include_self=True)
-        return m.cpu().numpy()
+        """max |x| of every problem (the divergence test), one native
+        segmented reduction over the contiguous per-problem segments."""
+        if getattr(self, "_xmax_d", None) is None:
+            self._xmax_d = torch.zeros(self.K, dtype=torch.float64, device=self.x.device)
+            self._xmax_h = np.zeros(self.K)
+        h = self._xmax_h
+        _lib.check(_lib.lib().amigo_segment_absmax(
+            _p(self.x), _p(self.be.xseg_d), self.K, _p(self._xmax_d),
+            h.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _s()))
+        return h.copy()
 
     def _check_convergence(self, i, options, A, res_norm, kkt):
         tol = options["convergence_tolerance"]

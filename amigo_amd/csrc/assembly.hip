@@ -13,6 +13,8 @@
 // (include/cuda/component_group_backend.cuh:37-120).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <cmath>
 #include <string>
 
@@ -24,6 +26,12 @@ constexpr int NT = 256;
 thread_local std::string g_asm_err;
 
 // ---- hyper-dual scalar for second derivatives of small nonlinear terms ----
+// The hyper-dual arithmetic (cart-pole dynamics) is compiled without
+// contraction of products into fused multiply-adds: the first-derivative
+// part of each operation is then the same IEEE sequence whichever direction
+// carries it and whichever other parts are known zero, so a gradient-only
+// evaluation reproduces the gradient of the Hessian evaluation bit for bit.
+#pragma clang fp contract(off)
 struct HD {
   double v, a, b, ab;  // value, d/da, d/db, d2/dadb
 };
@@ -48,6 +56,7 @@ __device__ __forceinline__ HD hcos(HD x) {
   const double s = sin(x.v), c = cos(x.v);
   return {c, -s * x.a, -s * x.b, -s * x.ab - c * x.a * x.b};
 }
+#pragma clang fp contract(fast)
 
 // ---- element types --------------------------------------------------------
 // AMIGO_ELEM_LINK: inputs (x, lambda); c = x.  L = lambda x.
@@ -132,6 +141,7 @@ __device__ void eval_elliptic(const double* z, const double* d, double alpha, do
 //   D = mc + mp sin^2(th)
 //   xdd = (u + mp sin(th) (l w^2 + g cos(th))) / D
 //   thdd = (-u cos(th) - mp l w^2 sin(th) cos(th) - (mc + mp) g sin(th)) / (l D)
+#pragma clang fp contract(off)
 template <class T>
 __device__ void cp_accel(T th, T w, T u, double mc, double mp, double l, T& xdd, T& thdd) {
   const double gr = 9.81;
@@ -146,18 +156,26 @@ __device__ void cp_accel(T th, T w, T u, double mc, double mp, double l, T& xdd,
 // data (dt, mc, mp, l).
 // f_e = (u0^2 + u1^2) dt / 2
 // c = s1 - s0 - dt/2 (f(s0, u0) + f(s1, u1)),  f = (xd, w, xdd, thdd)
+// WANT_H = false (gradient only): one hyper-dual direction per variable
+// instead of one pair per Hessian entry.  The first-derivative part of every
+// hyper-dual operation reads only values and first-direction parts, so the
+// gradient is the same as the one the Hessian path computes.
+template <bool WANT_H>
 __device__ void eval_cartpole(const double* z, const double* d, double alpha, double& L,
                               double* g, double* H) {
   constexpr int NC = 14;
   const double dt = d[0], mc = d[1], mp = d[2], l = d[3], h = 0.5 * dt;
-  for (int q = 0; q < NC * NC; q++) H[q] = 0.0;
+  if constexpr (WANT_H)
+    for (int q = 0; q < NC * NC; q++) H[q] = 0.0;
   for (int q = 0; q < NC; q++) g[q] = 0.0;
   const double* lam = z + 10;
   L = alpha * (z[4] * z[4] + z[9] * z[9]) * dt * 0.5;
   g[4] += alpha * z[4] * dt;
   g[9] += alpha * z[9] * dt;
-  H[4 * NC + 4] += alpha * dt;
-  H[9 * NC + 9] += alpha * dt;
+  if constexpr (WANT_H) {
+    H[4 * NC + 4] += alpha * dt;
+    H[9 * NC + 9] += alpha * dt;
+  }
   // Linear part: lam . (s1 - s0) - h lam . ((xd0, w0) + (xd1, w1)) in rows 0,1.
   for (int r = 0; r < 4; r++) {
     const double cr_lin = z[5 + r] - z[r];
@@ -165,8 +183,10 @@ __device__ void eval_cartpole(const double* z, const double* d, double alpha, do
     g[5 + r] += lam[r];
     g[r] -= lam[r];
     g[10 + r] += cr_lin;
-    H[(10 + r) * NC + 5 + r] = H[(5 + r) * NC + 10 + r] = 1.0;
-    H[(10 + r) * NC + r] = H[r * NC + 10 + r] = -1.0;
+    if constexpr (WANT_H) {
+      H[(10 + r) * NC + 5 + r] = H[(5 + r) * NC + 10 + r] = 1.0;
+      H[(10 + r) * NC + r] = H[r * NC + 10 + r] = -1.0;
+    }
   }
   for (int e = 0; e < 2; e++) {
     const int o = 5 * e;  // offset of (x, th, xd, w, u) for this end
@@ -176,13 +196,28 @@ __device__ void eval_cartpole(const double* z, const double* d, double alpha, do
     g[o + 3] -= h * lam[1];
     g[10] -= h * z[o + 2];
     g[11] -= h * z[o + 3];
-    H[10 * NC + o + 2] -= h;
-    H[(o + 2) * NC + 10] -= h;
-    H[11 * NC + o + 3] -= h;
-    H[(o + 3) * NC + 11] -= h;
+    if constexpr (WANT_H) {
+      H[10 * NC + o + 2] -= h;
+      H[(o + 2) * NC + 10] -= h;
+      H[11 * NC + o + 3] -= h;
+      H[(o + 3) * NC + 11] -= h;
+    }
     // rows 2,3: nonlinear in (th, w, u) = inputs (o+1, o+3, o+4).
     const int vi[3] = {o + 1, o + 3, o + 4};
     double val2 = 0.0, val3 = 0.0, g2[3], g3[3], h2[3][3], h3[3][3];
+    if constexpr (!WANT_H) {
+      for (int p = 0; p < 3; p++) {
+        HD th = hd(z[o + 1]), w = hd(z[o + 3]), u = hd(z[o + 4]);
+        HD* vars[3] = {&th, &w, &u};
+        vars[p]->a = 1.0;
+        HD xdd, thdd;
+        cp_accel(th, w, u, mc, mp, l, xdd, thdd);
+        val2 = xdd.v;
+        val3 = thdd.v;
+        g2[p] = xdd.a;
+        g3[p] = thdd.a;
+      }
+    } else
     for (int p = 0; p < 3; p++) {
       for (int q = p; q < 3; q++) {
         HD th = hd(z[o + 1]), w = hd(z[o + 3]), u = hd(z[o + 4]);
@@ -206,6 +241,7 @@ __device__ void eval_cartpole(const double* z, const double* d, double alpha, do
     g[13] -= h * val3;
     for (int p = 0; p < 3; p++) {
       g[vi[p]] -= h * (lam[2] * g2[p] + lam[3] * g3[p]);
+      if constexpr (!WANT_H) continue;
       H[12 * NC + vi[p]] -= h * g2[p];
       H[vi[p] * NC + 12] -= h * g2[p];
       H[13 * NC + vi[p]] -= h * g3[p];
@@ -216,7 +252,9 @@ __device__ void eval_cartpole(const double* z, const double* d, double alpha, do
   }
 }
 
-template <int NC>
+#pragma clang fp contract(fast)
+
+template <int NC, bool WANT_H>
 __device__ void eval_elem(int type, const double* z, const double* d, double alpha,
                           double& L, double* g, double* H) {
   if constexpr (NC == 2) {
@@ -228,7 +266,7 @@ __device__ void eval_elem(int type, const double* z, const double* d, double alp
   } else if constexpr (NC == 9) {
     eval_elliptic<3>(z, d, alpha, L, g, H);
   } else {
-    eval_cartpole(z, d, alpha, L, g, H);
+    eval_cartpole<WANT_H>(z, d, alpha, L, g, H);
   }
 }
 
@@ -237,13 +275,13 @@ __device__ void eval_elem(int type, const double* z, const double* d, double alp
 // alpha_blk, block b scales the objective by alpha_blk[b] (a batch of
 // problems, each padded to whole blocks); an element whose first input is
 // AMIGO_ELEM_PAD is padding and contributes nothing.
-template <int NC>
+template <int NC, bool WANT_H>
 __global__ __launch_bounds__(NT) void element_kernel(int type, int E, const int* __restrict__ idx,
                                                      const double* __restrict__ data, int ndata,
                                                      const double* __restrict__ x, double alpha,
                                                      const double* __restrict__ alpha_blk,
                                                      double* __restrict__ gbuf,
-                                                     double* __restrict__ hbuf, int want_h,
+                                                     double* __restrict__ hbuf,
                                                      double* __restrict__ lpart) {
   __shared__ double red[NT / 64];
   const int e = blockIdx.x * NT + threadIdx.x;
@@ -256,12 +294,12 @@ __global__ __launch_bounds__(NT) void element_kernel(int type, int E, const int*
       const int k = idx[(size_t)e * NC + q];
       z[q] = k >= 0 ? x[k] : 0.0;
     }
-    eval_elem<NC>(type, z, data + (size_t)e * ndata, alpha, Lval, g, H);
+    eval_elem<NC, WANT_H>(type, z, data + (size_t)e * ndata, alpha, Lval, g, H);
     if (gbuf) {
 #pragma unroll
       for (int q = 0; q < NC; q++) gbuf[(size_t)e * NC + q] = g[q];
     }
-    if (want_h) {
+    if constexpr (WANT_H) {
 #pragma unroll
       for (int q = 0; q < NC * NC; q++) hbuf[(size_t)e * NC * NC + q] = H[q];
     }
@@ -353,27 +391,31 @@ int amigo_elements_eval_scaled(int type, int E, const int* idx, const double* da
   if (E <= 0) return 0;
   const int nb = (E + NT - 1) / NT;
   hipStream_t s = (hipStream_t)stream;
-  const int want_h = hbuf != nullptr;
+  const bool want_h = hbuf != nullptr;
+  auto launch = [&](auto nc) {
+    constexpr int NC = decltype(nc)::value;
+    if (want_h)
+      hipLaunchKernelGGL((element_kernel<NC, true>), dim3(nb), dim3(NT), 0, s, type, E, idx,
+                         data, ndata, x, alpha, alpha_blk, gbuf, hbuf, lpart);
+    else
+      hipLaunchKernelGGL((element_kernel<NC, false>), dim3(nb), dim3(NT), 0, s, type, E, idx,
+                         data, ndata, x, alpha, alpha_blk, gbuf, hbuf, lpart);
+  };
   switch (type) {
     case AMIGO_ELEM_LINK:
-      hipLaunchKernelGGL(element_kernel<2>, dim3(nb), dim3(NT), 0, s, type, E, idx, data, ndata, x,
-                         alpha, alpha_blk, gbuf, hbuf, want_h, lpart);
+      launch(std::integral_constant<int, 2>{});
       break;
     case AMIGO_ELEM_DI_INTERVAL:
-      hipLaunchKernelGGL(element_kernel<8>, dim3(nb), dim3(NT), 0, s, type, E, idx, data, ndata, x,
-                         alpha, alpha_blk, gbuf, hbuf, want_h, lpart);
+      launch(std::integral_constant<int, 8>{});
       break;
     case AMIGO_ELEM_ELLIPTIC2D:
-      hipLaunchKernelGGL(element_kernel<7>, dim3(nb), dim3(NT), 0, s, type, E, idx, data, ndata, x,
-                         alpha, alpha_blk, gbuf, hbuf, want_h, lpart);
+      launch(std::integral_constant<int, 7>{});
       break;
     case AMIGO_ELEM_ELLIPTIC3D:
-      hipLaunchKernelGGL(element_kernel<9>, dim3(nb), dim3(NT), 0, s, type, E, idx, data, ndata, x,
-                         alpha, alpha_blk, gbuf, hbuf, want_h, lpart);
+      launch(std::integral_constant<int, 9>{});
       break;
     case AMIGO_ELEM_CARTPOLE:
-      hipLaunchKernelGGL(element_kernel<14>, dim3(nb), dim3(NT), 0, s, type, E, idx, data, ndata, x,
-                         alpha, alpha_blk, gbuf, hbuf, want_h, lpart);
+      launch(std::integral_constant<int, 14>{});
       break;
     default:
       g_asm_err = "unknown element type";

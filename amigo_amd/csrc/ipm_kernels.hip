@@ -999,6 +999,32 @@ int grid_for(int n) { return nblocks(n); }
 
 }  // namespace
 
+// Largest |x_i| of each problem's segment [seg[k], seg[k+1]) -- the
+// divergence test of convergence_check.py:82-83.  One workgroup per problem;
+// a maximum is exact, so the result does not depend on the reduction order,
+// and a NaN propagates as in np.max.
+__device__ __forceinline__ double nan_max(double a, double b) {
+  return (b > a || isnan(b)) && !isnan(a) ? b : a;
+}
+
+__global__ __launch_bounds__(NT) void segment_absmax_kernel(const double* __restrict__ x,
+                                                            const int* __restrict__ seg,
+                                                            double* __restrict__ out) {
+  __shared__ double red[NT / 64];
+  const int k = blockIdx.x;
+  double m = 0.0;
+  for (int i = seg[k] + threadIdx.x; i < seg[k + 1]; i += NT) m = nan_max(m, fabs(x[i]));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = red[0];
+    for (int w = 1; w < NT / 64; w++) r = nan_max(r, red[w]);
+    out[k] = r;
+  }
+}
+
 extern "C" {
 
 const char* amigo_ipm_last_error(void) { return g_ipm_err.c_str(); }
@@ -1452,6 +1478,23 @@ int amigo_ipm_batch_ir_update(const amigo_ipm_batch* B, const amigo_ipm_problem*
   hipLaunchKernelGGL(ir_update_kernel, dim3(B->nblk_total), dim3(NT), 0, (hipStream_t)stream,
                      bt_total(B), *P, *S, 0, corr, ezl, ezu, px, dzl, dzu);
   LAUNCH_CHECK("ir_update_kernel");
+  return 0;
+}
+
+int amigo_segment_absmax(const double* x, const int* seg, int K, double* out_dev,
+                         double* out_host, void* stream) {
+  if (K <= 0) return 0;
+  if (x == nullptr || seg == nullptr || out_dev == nullptr) return -1;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(segment_absmax_kernel, dim3(K), dim3(NT), 0, s, x, seg, out_dev);
+  LAUNCH_CHECK("segment_absmax_kernel");
+  if (out_host) {
+    hipError_t e = hipMemcpyAsync(out_host, out_dev, sizeof(double) * (size_t)K,
+                                  hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return set_err("hipMemcpyAsync", e);
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return set_err("hipStreamSynchronize", e);
+  }
   return 0;
 }
 

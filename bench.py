@@ -50,6 +50,9 @@ def parse():
                     help="skip the other-precision and serving measurements")
     ap.add_argument("--forms", default="precision,serving,batching_slots,batching,serving_lockstep",
                     help="comma-separated forms to measure")
+    ap.add_argument("--forms-only", action="store_true",
+                    help="measure only the --forms other than precision (no headline solve; "
+                         "for kernel profiles of one form)")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="MiB of one device copy run first (counter calibration)")
     ap.add_argument("--serve-requests", type=int, default=32)
@@ -82,6 +85,10 @@ def main():
     from amigo_amd.problems import elliptic
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.forms_only:
+        args.forms = ",".join(f for f in args.forms.split(",") if f and f != "precision")
+        print(json.dumps({"forms": forms(args, None, None, None, None)}))
+        return
     prob = elliptic.build_2d(m=args.m, seed=args.seed)
     opt = Optimizer(prob, order=args.order, precision=args.precision, scaling=not args.no_scaling)
     ldl = opt.solver.ldl

@@ -243,6 +243,13 @@ int amigo_ipm_batch_ir_update(const amigo_ipm_batch* B, const amigo_ipm_problem*
                               const amigo_ipm_state* S, const double* corr, const double* ezl,
                               const double* ezu, double* px, double* dzl, double* dzu,
                               void* stream);
+/* Largest |x_i| of each of K contiguous segments [seg[k], seg[k+1]) of x
+ * (the divergence test, amigo/algorithm/convergence_check.py:82-83, per
+ * problem of a batch; NaN propagates as in np.max).  seg: K + 1 device ints;
+ * out_dev: K device doubles; out_host (optional): K host doubles, written
+ * once the stream has finished. */
+int amigo_segment_absmax(const double* x, const int* seg, int K, double* out_dev,
+                         double* out_host, void* stream);
 /* OptVector::copy (include/interior_point_optimizer.h:21-90) of the active
  * problems' segments: what = 1 x only, 2 zl/zu/sl/su only, 3 all. */
 int amigo_ipm_batch_copy_state(const amigo_ipm_batch* B, const amigo_ipm_state* dst,

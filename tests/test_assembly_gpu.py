@@ -65,3 +65,26 @@ def test_assembly_is_deterministic():
     prob.gradient(1.0, x, g1)
     prob.gradient(1.0, x, g2)
     assert torch.equal(m1.data, m2.data) and torch.equal(g1, g2)
+
+
+@pytest.mark.parametrize("name", ["double_integrator", "elliptic2d", "elliptic3d", "cart_pole"])
+def test_gradient_only_evaluation_equals_the_hessian_evaluations_gradient(name):
+    """The element kernels' gradient-only variant (no Hessian wanted) gives
+    bitwise the gradient entries of the variant that also forms the
+    Hessian."""
+    from amigo_amd import _lib
+    from amigo_amd.problem import current_stream_ptr
+
+    prob = _problems()[name]()
+    x = torch.tensor(np.random.default_rng(7).normal(size=prob.n), device="cuda")
+    s = _lib.vp(current_stream_ptr())
+    for grp in prob.groups:
+        g1 = torch.zeros(grp.E * 16, dtype=torch.float64, device="cuda")
+        g2 = torch.zeros_like(g1)
+        hb = torch.zeros(grp.E * 16 * 16, dtype=torch.float64, device="cuda")
+        for gb, h in ((g1, _lib.vp(0)), (g2, _lib.dptr(hb))):
+            _lib.check(_lib.lib().amigo_elements_eval(
+                grp.etype, grp.E, _lib.dptr(grp.idx_d), _lib.dptr(grp.data_d), grp.nd,
+                _lib.dptr(x), 0.37, _lib.dptr(gb), h, _lib.vp(0), s), "asm")
+        torch.cuda.synchronize()
+        assert torch.equal(g1, g2), grp.etype

@@ -207,3 +207,34 @@ def test_concurrent_factorizations_on_separate_streams_repeat_bitwise():
         for r, (x, inert) in enumerate(xs):
             assert inert == in0, (t, r)
             assert np.array_equal(x, x0), (t, r, float(np.max(np.abs(x - x0))))
+
+
+def test_segment_absmax_equals_numpy_max_per_segment():
+    """The per-problem divergence-test reduction: max |x| of each segment,
+    exact, NaN propagating as in np.max, 0 for an empty segment."""
+    import ctypes
+
+    from amigo_amd import _lib
+    from amigo_amd.batch import _p, _s
+
+    rng = np.random.default_rng(0)
+    lens = [0, 1, 5, 256, 1000, 3, 4097, 64]
+    seg = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    x = rng.normal(size=seg[-1]) * 10.0 ** rng.integers(-5, 5, size=seg[-1])
+    x[seg[4] + 17] = np.nan
+    x[seg[6] + 4000] = -1e300
+    xd = torch.tensor(x, device="cuda")
+    sd = torch.tensor(seg, device="cuda")
+    out = torch.zeros(len(lens), dtype=torch.float64, device="cuda")
+    h = np.zeros(len(lens))
+    _lib.check(_lib.lib().amigo_segment_absmax(_p(xd), _p(sd), len(lens), _p(out),
+                                                h.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                _s()))
+    for k in range(len(lens)):
+        s = x[seg[k]:seg[k + 1]]
+        exp = np.max(np.abs(s)) if len(s) else 0.0
+        if np.isnan(exp):
+            assert np.isnan(h[k]), k
+        else:
+            assert h[k] == exp, k
+    assert np.array_equal(out.cpu().numpy(), h, equal_nan=True)
