@@ -1573,6 +1573,9 @@ class BatchOptimizer:
         self.wd_px = torch.zeros_like(self.px)
         active = np.ones(K, bool)
         self.fallback[:] = False
+        self._alone = {}
+        self._alone_futs = {}
+        self._alone_pool = None
         record_obj = options.get("record_objective", True)
         self.iterations_run = 0
         self.active_history = []
@@ -1676,11 +1679,14 @@ class BatchOptimizer:
                 filt_resets[reset] += 1
                 filt_rej[reset] = 0
             # Feasibility restoration is not batched: those problems leave
-            # the batch and are solved again alone.
+            # the batch and are solved again alone, starting now, beside the
+            # batch's remaining iterations.
             rest = B & ~accepted
             if rest.any():
                 self.fallback |= rest
                 active &= ~rest
+                for k in np.nonzero(rest)[0]:
+                    self._start_alone(int(k), options, on_done)
             G = acc_m
             if G.any():
                 eps = np.finfo(np.float64).eps
@@ -1714,34 +1720,44 @@ class BatchOptimizer:
                 data[k]["num_iterations"] = len(data[k]["iterations"]) - 1
                 data[k]["batched"] = True
                 on_done(int(k), data[k], self._solution_of(int(k)))
-        F = np.nonzero(self.fallback)[0]
-        self._alone = {}
-        if F.size:
-            # The problems that left the batch are solved again on their own,
-            # several at once on their own streams (no graph capture beside
-            # other threads' work).
-            import concurrent.futures as cf
-
-            def one(k):
-                stream = torch.cuda.Stream(self.bp.device)
-                with torch.cuda.stream(stream):
-                    o = Optimizer(self.problems[k], order=self.order, scaling=self.scaling,
-                                  precision=self.precision)
-                    o.solver.ldl.set_graphs(False)
-                    d = o.optimize(options)
-                    stream.synchronize()
-                d["solver_inertia_history"] = list(o.solver.inertia_history)
-                return k, o, d
-
-            with cf.ThreadPoolExecutor(max_workers=min(8, F.size)) as ex:
-                for k, o, d in ex.map(one, [int(k) for k in F]):
-                    d["batched"] = False
-                    data[k] = d
-                    self._alone[k] = o
-                    if on_done is not None:
-                        on_done(k, d, o.get_solution())
+        # The problems that left the batch: their alone solves (started when
+        # they left) replace the batch records.
+        for k, fut in sorted(self._alone_futs.items()):
+            _, o, d = fut.result()
+            d["batched"] = False
+            data[k] = d
+            self._alone[k] = o
+        if self._alone_pool is not None:
+            self._alone_pool.shutdown(wait=True)
+            self._alone_pool = None
         self.total_time = time.perf_counter() - start
         return data
+
+    def _start_alone(self, k, options, on_done):
+        """Solve problem k again on its own (a thread with its own stream,
+        no graph capture beside the batch's work); on_done, if given, gets
+        its result as soon as it finishes."""
+        import concurrent.futures as cf
+
+        if self._alone_pool is None:
+            self._alone_pool = cf.ThreadPoolExecutor(max_workers=8)
+        opts = dict(options)
+
+        def one():
+            stream = torch.cuda.Stream(self.bp.device)
+            with torch.cuda.stream(stream):
+                o = Optimizer(self.problems[k], order=self.order, scaling=self.scaling,
+                              precision=self.precision)
+                o.solver.ldl.set_graphs(False)
+                d = o.optimize(opts)
+                stream.synchronize()
+                d["solver_inertia_history"] = list(o.solver.inertia_history)
+                d["batched"] = False
+                if on_done is not None:
+                    on_done(k, d, o.get_solution())
+            return k, o, d
+
+        self._alone_futs[k] = self._alone_pool.submit(one)
 
     def _solver_history(self, k):
         """Inertia of every factorization of problem k, as the

@@ -452,6 +452,15 @@ def lockstep_batching(args):
     torch.cuda.synchronize()
     seq_rate = ns / (time.perf_counter() - t)
     rate = K / t_solve
+    # The problems that left the batch for restoration (solved alone beside
+    # the batch) against a fresh one-at-a-time solve of the same precision.
+    left = [int(k) for k in np.nonzero(bo.fallback)[0]]
+    left_same = True
+    for k in left:
+        o = Optimizer(probs[k], order="ND", scaling=True, precision="mixed")
+        d = o.optimize(opts)
+        left_same &= (d["num_iterations"] == out[k]["num_iterations"]
+                      and bool(np.array_equal(o.get_solution(), xs[k])))
     return {
         "problems": K,
         "workload": "cart-pole swing-up, N~U{16..512}, T=2 s, mc~U(0.5,2), mp~U(0.1,0.5), l~U(0.3,1)",
@@ -467,6 +476,7 @@ def lockstep_batching(args):
         "iterations_max": int(its.max()),
         "lockstep_iterations": int(bo.iterations_run),
         "left_batch_for_restoration": int(bo.fallback.sum()),
+        "left_batch_identical_to_alone": left_same,
         "batch_factorizations": int(bo.solver.n_factor),
         "kkt_rows_total": int(bo.bp.n),
         "fp64_fallbacks": int(bo.fp64_fallbacks),
