@@ -483,6 +483,11 @@ class BatchIPM:
         self.out_h = np.zeros(4 * K)
         self.act_d = torch.ones(K, dtype=torch.int32, device=dev)
         self._sc = [torch.zeros(K, dtype=torch.float64, device=dev) for _ in range(4)]
+        # Host copies of the last uploads: an upload that would write the
+        # same bits is skipped (each upload from pageable memory waits for
+        # the stream).
+        self._act_last = None
+        self._sc_last = [None] * 4
         self.constr_scale = torch.ones(self.nc_, dtype=torch.float64, device=dev)
         self.scale_vec = torch.ones(self.n, dtype=torch.float64, device=dev)
         self.scaling_active = False
@@ -506,7 +511,10 @@ class BatchIPM:
         if mask is None:
             b.active = None
         else:
-            self.act_d.copy_(torch.from_numpy(np.ascontiguousarray(mask, dtype=np.int32)))
+            m = np.ascontiguousarray(mask, dtype=np.int32)
+            if self._act_last is None or not np.array_equal(m, self._act_last):
+                self.act_d.copy_(torch.from_numpy(m))
+                self._act_last = m.copy()
             b.active = self.act_d.data_ptr()
 
     def B(self, mask=None):
@@ -514,9 +522,14 @@ class BatchIPM:
         return ctypes.byref(self._batch)
 
     def sc(self, slot, v):
-        """Device copy of K per-problem scalars."""
+        """Device copy of K per-problem scalars (uploaded only when its bits
+        differ from the slot's last upload)."""
         t = self._sc[slot]
-        t.copy_(torch.from_numpy(np.array(np.broadcast_to(np.asarray(v, np.float64), (self.K,)))))
+        h = np.array(np.broadcast_to(np.asarray(v, np.float64), (self.K,)))
+        last = self._sc_last[slot]
+        if last is None or not np.array_equal(h.view(np.int64), last.view(np.int64)):
+            t.copy_(torch.from_numpy(h))
+            self._sc_last[slot] = h
         return t
 
     def _o(self):
@@ -923,6 +936,15 @@ def _pow(a, e):
     return np.array([_pow1(float(x), e) for x in np.asarray(a, np.float64).ravel()]).reshape(np.shape(a))
 
 
+def _pow_at(a, e, I):
+    """_pow at the indices I only (zeros elsewhere): the scalar pow costs a
+    Python call per element, so it is taken only where a decision reads it."""
+    out = np.zeros(np.shape(a))
+    for i in I:
+        out[i] = _pow1(float(a[i]), e)
+    return out
+
+
 def _log10(a):
     with np.errstate(all="ignore"):
         return np.array([np.log10(float(x)) for x in np.asarray(a, np.float64).ravel()]).reshape(np.shape(a))
@@ -1121,10 +1143,13 @@ class BatchOptimizer:
 
     # ---- barrier (barrier_strategy/heuristic.py) ------------------------------
     @staticmethod
-    def _loqo(xi, comp, gamma, r, tol):
+    def _loqo(xi, comp, gamma, r, tol, M):
+        """The LOQO barrier rule for the problems of the mask M (the other
+        entries are not used)."""
         with np.errstate(divide="ignore", invalid="ignore"):
             term = (1 - r) * (1 - xi) / np.where(xi > 1e-10, xi, 1.0)
-        factor = np.where(xi > 1e-10, _pow(np.minimum(term, 2.0), 3), 2.0**3)
+        p = _pow_at(np.minimum(term, 2.0), 3, np.nonzero(M & (xi > 1e-10))[0])
+        factor = np.where(xi > 1e-10, p, 2.0**3)
         return np.maximum(gamma * factor * comp, tol)
 
     def _barrier_update(self, i, res_norm, options, A):
@@ -1139,7 +1164,8 @@ class BatchOptimizer:
             raise NotImplementedError("the batched driver runs the heuristic barrier strategy")
         comp_h, xi_h = self.be.complementarity(self.vars, M)
         mu_floor = min(tol, cit) / (options["barrier_tol_factor"] + 1.0)
-        new = self._loqo(xi_h, comp_h, options["heuristic_barrier_gamma"], options["heuristic_barrier_r"], mu_floor)
+        new = self._loqo(xi_h, comp_h, options["heuristic_barrier_gamma"], options["heuristic_barrier_r"],
+                         mu_floor, M)
         self.mu[M] = new[M]
 
     def _increase_barrier_on_rejections(self, I, max_rej, inc, initial):
@@ -1318,14 +1344,16 @@ class BatchOptimizer:
         theta_max = 1e4 * np.maximum(1.0, theta_0)
         neg = ref_dphi < 0.0
         nd = np.where(neg, -ref_dphi, 1.0)
+        # The powers are read only for the searching problems with
+        # ref_dphi < 0 (the single-problem tests short-circuit otherwise).
+        SN = S[neg[S]]
+        nd_sphi = _pow_at(nd, s_phi, SN)
+        rt_stheta = _pow_at(ref_theta, s_theta, SN)
         with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
             am = np.where(neg, np.minimum(gamma_theta, gamma_phi * ref_theta / nd), gamma_theta)
             am = np.where(neg & (ref_theta <= theta_min),
-                          np.minimum(am, delta * _pow(ref_theta, s_theta) / _pow(nd, s_phi)), am)
+                          np.minimum(am, delta * rt_stheta / nd_sphi), am)
         alpha_min = np.where(wd, alpha_x, am * 0.05)
-
-        nd_sphi = _pow(nd, s_phi)
-        rt_stheta = _pow(ref_theta, s_theta)
 
         def is_ftype(a):
             with np.errstate(invalid="ignore", over="ignore"):
@@ -1334,13 +1362,17 @@ class BatchOptimizer:
         def armijo(tb, a):
             return (tb - ref_barr) - eta_phi * a * ref_dphi <= EPS10 * np.abs(ref_barr)
 
-        def acceptable_to_iterate(tb, tt):
+        def acceptable_to_iterate(tb, tt, m):
+            """For the problems of mask m (other entries are not used)."""
             ok = np.ones(K, bool)
-            up = tb > ref_barr
+            # The logarithms only where the objective went up, as the
+            # single-problem test takes them.
             with np.errstate(divide="ignore", invalid="ignore"):
-                basval = np.where(np.abs(ref_barr) > 10.0, _log10(np.abs(ref_barr)), 1.0)
-                big = _log10(np.maximum(tb - ref_barr, 1e-300)) > obj_max_inc + basval
-            ok &= ~(up & big)
+                for i in np.nonzero(m & (tb > ref_barr))[0]:
+                    rb = abs(float(ref_barr[i]))
+                    basval = np.log10(rb) if rb > 10.0 else 1.0
+                    if np.log10(max(float(tb[i]) - float(ref_barr[i]), 1e-300)) > obj_max_inc + basval:
+                        ok[i] = False
             dec = ((tt - (1.0 - gamma_theta) * ref_theta <= EPS10 * np.abs(ref_theta))
                    | ((tb - ref_barr) - (-gamma_phi * ref_theta) <= EPS10 * np.abs(ref_barr)))
             return ok & dec
@@ -1356,7 +1388,7 @@ class BatchOptimizer:
             m = self._mx(I)
             tm = m & ~(tt > theta_max)
             fa = tm & (a > 0.0) & is_ftype(a) & (ref_theta <= theta_min)
-            pass1 = np.where(fa, armijo(tb, a), acceptable_to_iterate(tb, tt)) & tm
+            pass1 = np.where(fa, armijo(tb, a), acceptable_to_iterate(tb, tt, tm & ~fa)) & tm
             J = np.nonzero(pass1)[0]
             if J.size:
                 ok = self.filter.acceptable(J, tb[J], tt[J])
