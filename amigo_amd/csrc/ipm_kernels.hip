@@ -777,12 +777,11 @@ __global__ void spmv_kernel(int n, const int* __restrict__ rowp,
 __global__ void scale_csr_kernel(int n, const int* __restrict__ rowp,
                                  const int* __restrict__ cols,
                                  const double* __restrict__ sv, double* __restrict__ vals) {
-  const int lane = threadIdx.x & 63;
-  const int row = (blockIdx.x * NT + threadIdx.x) >> 6;
-  const int nw = (gridDim.x * NT) >> 6;
-  for (int r = row; r < n; r += nw) {
+  // One thread per row (KKT rows hold a few entries; each entry's product
+  // is the same whichever thread forms it).
+  for (int r = blockIdx.x * NT + threadIdx.x; r < n; r += gridDim.x * NT) {
     const double d = sv[r];
-    for (int p = rowp[r] + lane; p < rowp[r + 1]; p += 64) vals[p] *= d * sv[cols[p]];
+    for (int p = rowp[r]; p < rowp[r + 1]; p++) vals[p] *= d * sv[cols[p]];
   }
 }
 
@@ -1172,8 +1171,8 @@ int amigo_csr_spmv(int n, const int* rowp, const int* cols, const double* vals,
 
 int amigo_csr_scale_symmetric(int n, const int* rowp, const int* cols,
                               const double* sv, double* vals, void* stream) {
-  int nb = (n * 64 + NT - 1) / NT;
-  if (nb > 8192) nb = 8192;
+  int nb = (n + NT - 1) / NT;
+  if (nb > 65535) nb = 65535;
   if (nb < 1) nb = 1;
   hipLaunchKernelGGL(scale_csr_kernel, dim3(nb), dim3(NT), 0, (hipStream_t)stream, n, rowp,
                      cols, sv, vals);

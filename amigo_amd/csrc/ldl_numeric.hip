@@ -1656,23 +1656,20 @@ __global__ void equilibrate_kernel(int n, const int* __restrict__ rowp,
                                    double* __restrict__ sout,
                                    const int* __restrict__ row_blk,
                                    const int* __restrict__ blk_active) {
-  const int lane = threadIdx.x & 63;
-  const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int nw = (gridDim.x * blockDim.x) >> 6;
-  for (int r = w; r < n; r += nw) {
+  // One thread per row: KKT rows hold a few entries, and a maximum is exact
+  // in any order.
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
     // Rows of blocks left out keep their scale factors.
     if (row_blk && !blk_active[row_blk[r]]) continue;
     double m = 0.0;
     const double sr = sin ? sin[r] : 1.0;
-    for (int p = rowp[r] + lane; p < rowp[r + 1]; p += 64) {
+    for (int p = rowp[r]; p < rowp[r + 1]; p++) {
       const int c = cols[p];
       double v = vals[p];
       if (c == r && shift) v += shift[r];
       m = fmax(m, fabs(v * sr * (sin ? sin[c] : 1.0)));
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-    if (lane == 0) sout[r] = m > 0.0 && m < INFINITY ? sr / sqrt(m) : sr;
+    sout[r] = m > 0.0 && m < INFINITY ? sr / sqrt(m) : sr;
   }
 }
 
@@ -2558,8 +2555,8 @@ int LdlNumeric::enqueue_factor(const double* vals, const double* shift,
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, stream, S_);
   HIPCHK(hipGetLastError());
   if (scaling_) {
-    int nb = (n_ * 64 + NT - 1) / NT;
-    nb = std::min(std::max(nb, 1), 8192);
+    int nb = (n_ + NT - 1) / NT;
+    nb = std::min(std::max(nb, 1), 65535);
     // Five Ruiz passes, alternating between the two scale buffers, the
     // last one writing scale_.
     constexpr int kPasses = 5;
