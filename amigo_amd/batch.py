@@ -477,6 +477,10 @@ class BatchIPM:
         self.pseg_d = T(self.pseg, torch.int32)
         self.cseg_d = T(self.cseg, torch.int32)
         self.xseg_d = T(xs, torch.int32)
+        # The Hessian buffer of the NLP scaling (pattern built once, here).
+        self._scal_hess = bp.create_matrix()
+        self._segmax_d = torch.zeros(K, dtype=torch.float64, device=dev)
+        self._segmax_h = np.zeros(K)
         nbmax = max(self._tabs[0][0], self._tabs[1][0])
         self.ws = torch.zeros(4 * nbmax + 16, dtype=torch.float64, device=dev)
         self.out_d = torch.zeros(4 * K, dtype=torch.float64, device=dev)
@@ -647,19 +651,23 @@ class BatchIPM:
                                                            self._o(), _s()), "ipm")
         return self._rows(1)[:, 0]
 
+    def segmax(self, v, seg_d):
+        """max |v| over each problem's contiguous segment of v (host copy)."""
+        h = self._segmax_h
+        _lib.check(_lib.lib().amigo_segment_absmax(
+            _p(v), _p(seg_d), self.K, _p(self._segmax_d),
+            h.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _s()), "ipm")
+        return h.copy()
+
     # ---- NLP scaling (optimizer.h:502-598), per problem ----------------------
     def compute_nlp_scaling(self, x, grad, solver, max_gradient=100.0, min_value=1e-8):
-        K = self.K
-        g_obj = grad[self.pidx_g].abs()
-        obj_max = torch.zeros(K, dtype=torch.float64, device=self.device)
-        obj_max.scatter_reduce_(0, self.pprob, g_obj, reduce="amax", include_self=True)
-        obj_max = obj_max.cpu().numpy()
+        obj_max = self.segmax(grad[self.pidx_g].contiguous(), self.pseg_d)
         has_p = self.n_primal_k > 0
         with np.errstate(divide="ignore"):
             sc = np.where(has_p & (obj_max > max_gradient), max_gradient / np.where(obj_max > 0, obj_max, 1.0), 1.0)
         sc = np.where(sc < min_value, min_value, sc)
         self.obj_scale = sc
-        hess = self.bp.create_matrix()
+        hess = self._scal_hess
         self.bp.hessian(1.0, x, hess)
         rowp_d, _ = hess.pattern_device()
         dc = torch.ones(self.nc_, dtype=torch.float64, device=self.device)
@@ -1075,10 +1083,7 @@ class BatchOptimizer:
         # starts with this factorization.
         self._ls_inertia = [(int(p[k]), int(q[k])) if nan[k] == 0 else None for k in range(self.K)]
         self.solver.solve(self.res, self.px)
-        lam = self.px[be.cidx_g].abs()
-        lmax = torch.zeros(self.K, dtype=torch.float64, device=self.x.device)
-        lmax.scatter_reduce_(0, be.cprob, lam, reduce="amax", include_self=True)
-        lmax = lmax.cpu().numpy()
+        lmax = be.segmax(self.px[be.cidx_g].contiguous(), be.cseg_d)
         reject = (self.n_dual > 0) & (lmax > lambda_max)
         keep = torch.as_tensor(~reject, device=self.x.device)[be.cprob]
         ci = be.cidx_g

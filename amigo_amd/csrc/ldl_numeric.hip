@@ -610,6 +610,15 @@ __device__ __forceinline__ bool node_on(const DevState& S, int node) {
   return S.blk_active == nullptr || S.blk_active[S.node_blk[node]] != 0;
 }
 
+// The factorization's failure flag, read without a read-modify-write (many
+// workgroups check it; an atomic add of zero would serialize them at the
+// cache).  A flag set by an earlier launch is visible at the kernel
+// boundary; one set in the same launch only means the factorization is
+// redone.
+__device__ __forceinline__ int fail_flag(const DevState& S) {
+  return __hip_atomic_load(&S.flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // One workgroup per front of the level: allocate, assemble, and (unless the
 // front is routed to the blocked path) factor.
 __global__ __launch_bounds__(NT) void level_kernel(
@@ -643,7 +652,7 @@ __global__ __launch_bounds__(NT) void level_kernel(
     if (nd) atomicAdd(&sh[2], nd);
   }
   __syncthreads();
-  if (tid == 0 && atomicAdd(&S.flags[0], 0) == 0) {
+  if (tid == 0 && fail_flag(S) == 0) {
     int D = sh[2];
     const int m = ns + D + cs;
     if (dcap_big >= 0 && m > LDS_MAX_M && (!big_mask[node] || D > dcap_big)) {
@@ -837,12 +846,8 @@ __global__ __launch_bounds__(NT) void level_kernel(
     S.pstate[PS * node + 1] = np;
     S.pstate[PS * node + 3] = 0;
     S.pstate[PS * node + 4] = 0;
-    if (ns + D - np) atomicAdd(&S.flags[5], ns + D - np);
-    atomicMax(&S.flags[6], m);
-    if (cnt[0]) atomicAdd(&S.flags[1], cnt[0]);
-    if (cnt[1]) atomicAdd(&S.flags[2], cnt[1]);
-    if (cnt[2]) atomicAdd(&S.flags[3], cnt[2]);
-    if (cnt[3]) atomicAdd(&S.flags[4], cnt[3]);
+    // Inertia, delays and the largest front are summed over the nodes
+    // after the factorization (factor_stats_kernel).
     for (int q = 0; q < 4; q++) S.ninert[4 * node + q] = cnt[q];
   }
 }
@@ -855,7 +860,7 @@ __global__ __launch_bounds__(NT) void level_kernel(
 // (3) extend-add the children, one child index per launch (child order as in
 // sparse_ldl.h:1084-1152).
 __device__ __forceinline__ bool big_active(const DevState& S, int node) {
-  return node_on(S, node) && S.pstate[PS * node + 4] == 1 && atomicAdd(&S.flags[0], 0) == 0;
+  return node_on(S, node) && S.pstate[PS * node + 4] == 1 && fail_flag(S) == 0;
 }
 
 __global__ __launch_bounds__(NT) void big_zero_kernel(DevPlan P, DevState S,
@@ -1301,7 +1306,7 @@ __global__ __launch_bounds__(REG ? NTH : NTP) void panel_kernel(DevState S,
   __shared__ int bad;
   int* cnt = psh.cnt;
   const int tid = threadIdx.x;
-  if (tid == 0) bad = atomicAdd(&S.flags[0], 0);
+  if (tid == 0) bad = fail_flag(S);
   __syncthreads();
   if (bad != 0) return;
   if (!node_on(S, node)) return;
@@ -1478,10 +1483,6 @@ __global__ __launch_bounds__(REG ? NTH : NTP) void panel_kernel(DevState S,
     ps[1] = nc;
     ps[2] = kstart;
     ps[3] = kw;
-    if (cnt[0]) atomicAdd(&S.flags[1], cnt[0]);
-    if (cnt[1]) atomicAdd(&S.flags[2], cnt[1]);
-    if (cnt[2]) atomicAdd(&S.flags[3], cnt[2]);
-    if (cnt[3]) atomicAdd(&S.flags[4], cnt[3]);
     for (int q = 0; q < 4; q++)
       if (cnt[q]) atomicAdd(&S.ninert[4 * node + q], cnt[q]);
   }
@@ -1519,7 +1520,7 @@ __global__ __launch_bounds__(NT) void trailing_mfma_kernel(DevState S,
   __shared__ T As[TILE][TK_PAD + 1];
   __shared__ T Bs[TILE][TK_PAD + 1];
   __shared__ int bad;
-  if (threadIdx.x == 0) bad = atomicAdd(&S.flags[0], 0);
+  if (threadIdx.x == 0) bad = fail_flag(S);
   __syncthreads();
   if (bad != 0) return;
   if (!node_on(S, node)) return;
@@ -1606,7 +1607,7 @@ __global__ __launch_bounds__(NT) void trailing_mfma_kernel(DevState S,
 __global__ void finalize_big_kernel(DevState S, const int* __restrict__ bnodes,
                                     const int* __restrict__ bns, int count) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= count || atomicAdd(&S.flags[0], 0) != 0) return;
+  if (t >= count || fail_flag(S) != 0) return;
   const int node = bnodes[t];
   if (!node_on(S, node)) return;
   if (S.pstate[PS * node + 4] == 0) return;  // factored in a workgroup
@@ -1618,8 +1619,41 @@ __global__ void finalize_big_kernel(DevState S, const int* __restrict__ bnodes,
   const int nfs = bns[t] + S.D[node];
   S.npiv[node] = nc;
   S.ndel[node] = nfs - nc;
-  if (nfs - nc) atomicAdd(&S.flags[5], nfs - nc);
-  atomicMax(&S.flags[6], S.m[node]);
+}
+
+// flags[1..4] = inertia counts (+, -, 0, non-finite), flags[5] = delayed
+// pivots, flags[6] = largest front, over the fronts this factorization
+// took part in, from the per-node counts (integer sums: exact in any order).
+__global__ __launch_bounds__(NT) void factor_stats_kernel(DevState S, int K) {
+  __shared__ int red[6][NT / 64];
+  if (fail_flag(S) != 0) return;
+  int c[6] = {0, 0, 0, 0, 0, 0};
+  for (int node = blockIdx.x * NT + threadIdx.x; node < K; node += gridDim.x * NT) {
+    if (!node_on(S, node)) continue;
+    for (int q = 0; q < 4; q++) c[q] += S.ninert[4 * node + q];
+    c[4] += S.ndel[node];
+    c[5] = max(c[5], S.m[node]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < 5; q++) c[q] += __shfl_xor(c[q], off, 64);
+    c[5] = max(c[5], __shfl_xor(c[5], off, 64));
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int q = 0; q < 6; q++) red[q][w] = c[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int ww = 1; ww < NT / 64; ww++) {
+      for (int q = 0; q < 5; q++) c[q] += red[q][ww];
+      c[5] = max(c[5], red[5][ww]);
+    }
+    for (int q = 0; q < 4; q++)
+      if (c[q]) atomicAdd(&S.flags[1 + q], c[q]);
+    if (c[4]) atomicAdd(&S.flags[5], c[4]);
+    atomicMax(&S.flags[6], c[5]);
+  }
 }
 
 // Per-block inertia of a block-diagonal matrix: block b owns nodes
@@ -1707,7 +1741,7 @@ __global__ void reset_kernel(DevState S) {
 __global__ __launch_bounds__(NT) void fpos_kernel(DevPlan P, DevState S, int K) {
   // After a failed (to be redone) factorization the fronts are not all
   // allocated: nothing to map.
-  if (atomicAdd(&S.flags[0], 0) != 0) return;
+  if (fail_flag(S) != 0) return;
   for (int node = blockIdx.x; node < K; node += gridDim.x) {
     if (!node_on(S, node)) continue;
     const int ns = P.ns[node], D = S.D[node], nfs = ns + D;
@@ -2720,6 +2754,8 @@ int LdlNumeric::enqueue_factor(const double* vals, const double* shift,
     HIPCHK(hipGetLastError());
   }
   hipLaunchKernelGGL(fpos_kernel, dim3(std::min(K_, 4096)), dim3(NT), 0, stream, P_, S_, K_);
+  hipLaunchKernelGGL(factor_stats_kernel, dim3(std::max(1, std::min((K_ + NT - 1) / NT, 1024))),
+                     dim3(NT), 0, stream, S_, K_);
   HIPCHK(hipGetLastError());
   return 0;
 }
