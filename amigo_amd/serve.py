@@ -172,7 +172,8 @@ class BatchSolveServer:
     of the KKT factorization and every IPM kernel launch.  Up to
     `max_batches` batches run at once, each on its own HIP stream, so a new
     batch starts while an older one finishes its slowest problems.  A batch
-    takes at most `max_batch` requests and is admitted by predicted device
+    takes at most `max_batch` requests, by default of one problem type
+    (group_by_type), and is admitted by predicted device
     memory (the sum of its problems' SparseLDL.memory_bytes and vectors)
     against the budget.  Each request's result is returned when its own
     problem finishes, not when its batch does, and equals the request
@@ -184,7 +185,8 @@ class BatchSolveServer:
     """
 
     def __init__(self, max_batch=512, max_batches=2, gather_s=0.005, memory_budget=None,
-                 precision="mixed", options=None, device="cuda", scaling=True, order="ND"):
+                 precision="mixed", options=None, device="cuda", scaling=True, order="ND",
+                 group_by_type=True):
         self.device = torch.device(device)
         if memory_budget is None:
             free, _ = torch.cuda.mem_get_info(self.device)
@@ -196,6 +198,7 @@ class BatchSolveServer:
         self.options = dict(options or {})
         self.scaling = scaling
         self.order = order
+        self.group_by_type = bool(group_by_type)
         self._q = queue.Queue()
         self._cv = threading.Condition()
         self._inflight = 0
@@ -266,10 +269,28 @@ class BatchSolveServer:
                     stop = True
                     break
                 pending.append(item)
-            batch, pending = pending[: self.max_batch], pending[self.max_batch:]
+            batch, pending = self._take_batch(pending)
             t = threading.Thread(target=self._run_batch, args=(batch,), daemon=True)
             self._workers = [w for w in self._workers if w.is_alive()] + [t]
             t.start()
+
+    def _take_batch(self, pending):
+        """The next batch from the queued requests, in arrival order.  With
+        group_by_type, only requests of the oldest request's problem type:
+        a lockstep factorization takes as many level launches as its
+        deepest elimination tree, so a trajectory (a few levels) batched
+        with a grid problem (tens of levels) would pay for the grid's
+        depth at every iteration."""
+        if not self.group_by_type:
+            return pending[: self.max_batch], pending[self.max_batch:]
+        kind = pending[0][0].get("type")
+        take, rest = [], []
+        for item in pending:
+            if len(take) < self.max_batch and item[0].get("type") == kind:
+                take.append(item)
+            else:
+                rest.append(item)
+        return take, rest
 
     # ---- one batch -----------------------------------------------------------
     def _run_batch(self, batch):

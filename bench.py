@@ -375,7 +375,7 @@ def lockstep_serving(args):
         seq_m = [f.result() for f in [srv.submit(sp) for sp in specs[:ns]]]
     out = {"requests_per_rate": n,
            "mix": "70% cart-pole N~U{32..1024}, 30% 2-D elliptic side~U{32..192}",
-           "form": "lockstep batches of the requests queued while earlier batches run, up to 4 batches "
+           "form": "lockstep batches of the requests queued while earlier batches run (one problem type per batch), up to 4 batches "
                    "at once on their own streams, mixed precision; each result returned when its own "
                    "problem finishes",
            "sequential_fp64_sample": ns, "sequential_fp64_solves_per_s": seq_rate, "rates": []}
