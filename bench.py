@@ -39,7 +39,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--m", type=int, default=256)
+    ap.add_argument("--m", type=int, default=256, help="grid side of the elliptic workload")
+    ap.add_argument("--dim", type=int, default=2, choices=[2, 3],
+                    help="2-D (default; the headline workload) or 3-D elliptic control")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--order", default="ND")
@@ -89,7 +91,7 @@ def main():
         args.forms = ",".join(f for f in args.forms.split(",") if f and f != "precision")
         print(json.dumps({"forms": forms(args, None, None, None, None)}))
         return
-    prob = elliptic.build_2d(m=args.m, seed=args.seed)
+    prob = (elliptic.build_2d if args.dim == 2 else elliptic.build_3d)(m=args.m, seed=args.seed)
     opt = Optimizer(prob, order=args.order, precision=args.precision, scaling=not args.no_scaling)
     ldl = opt.solver.ldl
     opt.solver.keep_first = True
@@ -199,7 +201,8 @@ def main():
         "precision_form": args.precision,
         "data": "synthetic (seeded 16-mode target field)",
         "config": {
-            "workload": f"2-D semilinear elliptic control, {args.m}x{args.m} grid, alpha=1e-4, 0<=u<=50",
+            "workload": (f"{args.dim}-D semilinear elliptic control, "
+                         f"{'x'.join([str(args.m)] * args.dim)} grid, alpha=1e-4, 0<=u<=50"),
             "kkt_rows": int(prob.n),
             "kkt_nnz": int(prob.nnz),
             "ordering": args.order,
