@@ -238,3 +238,19 @@ def test_segment_absmax_equals_numpy_max_per_segment():
         else:
             assert h[k] == exp, k
     assert np.array_equal(out.cpu().numpy(), h, equal_nan=True)
+
+
+def test_batch_of_one_problem_equals_its_single_problem_solve():
+    from amigo_amd.batch import BatchOptimizer
+    from amigo_amd.optimizer import Optimizer
+    from amigo_amd.problems import cart_pole
+
+    p = cart_pole.seeded(321, nmin=40, nmax=200)
+    bo = BatchOptimizer([p], order="ND", scaling=True)
+    r = bo.optimize({"record_objective": False})[0]
+    o = Optimizer(p, order="ND", scaling=True)
+    d = o.optimize({"record_objective": False})
+    assert r["converged"] and d["converged"]
+    assert r["num_iterations"] == d["num_iterations"]
+    assert r["inertia_history"] == d["inertia_history"]
+    assert np.array_equal(bo.get_solution()[0], o.get_solution())

@@ -158,3 +158,24 @@ def test_union_rejects_mismatched_pattern():
     m = CSRMat(bp.n, bp.n, bp.rowp, bp.cols, None, device="cpu")
     with pytest.raises(AmigoError):
         SparseLDL.union(m, [problem_ldl(p, "ND") for p in probs[::-1]])
+
+
+def test_server_takes_one_problem_type_per_batch_in_arrival_order():
+    """BatchSolveServer._take_batch: the oldest request's type first, in
+    arrival order, at most max_batch; the rest stay queued in order."""
+    from types import SimpleNamespace
+
+    from amigo_amd.serve import BatchSolveServer
+
+    kinds = ["cart_pole", "elliptic", "cart_pole", "double_integrator", "cart_pole", "elliptic"]
+    pending = [({"type": t, "i": i}, None, float(i)) for i, t in enumerate(kinds)]
+    srv = SimpleNamespace(max_batch=2, group_by_type=True)
+    take, rest = BatchSolveServer._take_batch(srv, pending)
+    assert [it[0]["i"] for it in take] == [0, 2]
+    assert [it[0]["i"] for it in rest] == [1, 3, 4, 5]
+    take, rest = BatchSolveServer._take_batch(srv, rest)
+    assert [it[0]["i"] for it in take] == [1, 5]
+    assert [it[0]["i"] for it in rest] == [3, 4]
+    srv.group_by_type = False
+    take, rest = BatchSolveServer._take_batch(srv, pending)
+    assert [it[0]["i"] for it in take] == [0, 1] and len(rest) == 4

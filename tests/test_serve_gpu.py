@@ -93,3 +93,15 @@ def test_lockstep_poisson_stream():
         st = run_poisson(srv, specs, rate=200.0, seed=6)
     assert st["converged"] == len(specs)
     assert 0 < st["p50_ms"] <= st["p99_ms"]
+
+
+def test_lockstep_batches_hold_one_problem_type():
+    from amigo_amd.serve import BatchSolveServer
+
+    with BatchSolveServer(max_batch=8, max_batches=1, gather_s=0.05) as srv:
+        futs = [srv.submit(s) for s in SPECS]
+        res = [f.result() for f in futs]
+    assert all(r["converged"] for r in res)
+    assert sum(len(b) for b in srv.batch_specs) == len(SPECS)
+    for specs in srv.batch_specs:
+        assert len({s["type"] for s in specs}) == 1
