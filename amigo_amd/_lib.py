@@ -208,6 +208,10 @@ _SIGS.update(
         "amigo_ipm_initialize": (_I, [_V, _V, _I, _V, _D, _D, _D, _D, _V, _V, _PS, _I, _I, _V]),
         "amigo_ipm_floor_slacks": (_I, [_PP, _PS, _V, _V, _D, _D, _D, _V, _c_dbl_p, _V]),
         "amigo_csr_spmv": (_I, [_I, _V, _V, _V, _V, _V, _V]),
+        "amigo_csr_spmv_masked": (_I, [_I, _V, _V, _V, _V, _V, _V, _V, _V]),
+        "amigo_csr_scale_symmetric_masked": (_I, [_I, _V, _V, _V, _V, _V, _V, _V]),
+        "amigo_elements_eval_masked": (_I, [_I, _I, _V, _V, _I, _V, _D, _V, _V, _V, _V, _V, _V, _V]),
+        "amigo_gather_sum_masked": (_I, [_I, _V, _V, _V, _V, _I, _V, _I, _V, _V]),
         "amigo_csr_scale_symmetric": (_I, [_I, _V, _V, _V, _V, _V]),
         "amigo_ipm_constraint_scaling": (_I, [_I, _V, _V, _V, _D, _D, _V, _V]),
         "amigo_ipm_scale_rows": (_I, [_I, _V, _V, _V, _I, _V]),

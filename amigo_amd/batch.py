@@ -214,6 +214,7 @@ class BatchProblem:
             G.idx_d = T(G.idx, torch.int32)
             G.data_d = T(G.data, torch.float64)
             G.bprob_d = T(G.bprob, torch.int64)
+            G.bprob32_d = T(G.bprob, torch.int32)
             G.alpha_d = torch.ones(G.E // NT, dtype=torch.float64, device=dev)
         self.h_src_d = T(self.h_src, torch.int64)
         self.h_ptr_d = T(self.h_ptr, torch.int32)
@@ -227,6 +228,12 @@ class BatchProblem:
         self.lidx_d = T(self.lidx if self.lidx.size else np.zeros(1, np.int32), torch.int32)
         self.diag_idx_d = T(self.diag_idx, torch.int64)
         self._lhost = np.zeros(self.K)
+        # Output groups of the masked evaluations: gradient rows and Hessian
+        # slots of each problem (contiguous), and the device mask.
+        self.xseg64_d = T(self.xseg, torch.int64)
+        self.hseg64_d = T(self.rowp[self.xseg].astype(np.int64), torch.int64)
+        self._act_d = torch.zeros(self.K, dtype=torch.int32, device=dev)
+        self._act_last = None
 
     # ---- per-problem objective scale ---------------------------------------
     def set_obj_scales(self, alpha):
@@ -248,11 +255,24 @@ class BatchProblem:
     def get_multiplier_indicator(self):
         return self.mult_ind
 
-    def _eval(self, scaled, x, want_g, want_h, want_l):
+    def active_dev(self, mask):
+        """Device copy of a K-problem mask (None when every problem is in),
+        uploaded only when it changes."""
+        if mask is None or bool(np.all(mask)):
+            return None
+        m = np.ascontiguousarray(mask, dtype=np.int32)
+        if self._act_last is None or not np.array_equal(m, self._act_last):
+            self._act_d.copy_(torch.from_numpy(m))
+            self._act_last = m.copy()
+        return self._act_d
+
+    def _eval(self, scaled, x, want_g, want_h, want_l, mask=None):
         """scaled: True -> each problem's objective scale; a float -> that
-        scale for every problem (as the single-problem call passes it)."""
+        scale for every problem (as the single-problem call passes it).
+        mask: only those problems' elements (the others write nothing)."""
         L = _lib.lib()
         s = _s()
+        act = self.active_dev(mask)
         for G in self.ugroups:
             gb = _lib.vp(self.gbuf.data_ptr() + 8 * G.goff) if want_g else _lib.vp(0)
             hb = _lib.vp(self.hbuf.data_ptr() + 8 * G.hoff) if want_h else _lib.vp(0)
@@ -261,21 +281,40 @@ class BatchProblem:
                 alpha, ab = 1.0, _p(G.alpha_d)
             else:
                 alpha, ab = float(scaled), _lib.vp(0)
-            _lib.check(
-                L.amigo_elements_eval_scaled(G.etype, G.E, _p(G.idx_d), _p(G.data_d), G.nd, _p(x),
-                                             alpha, ab, gb, hb, lp, s),
-                "asm",
-            )
+            if act is None:
+                rc = L.amigo_elements_eval_scaled(G.etype, G.E, _p(G.idx_d), _p(G.data_d), G.nd, _p(x),
+                                                  alpha, ab, gb, hb, lp, s)
+            else:
+                rc = L.amigo_elements_eval_masked(G.etype, G.E, _p(G.idx_d), _p(G.data_d), G.nd, _p(x),
+                                                  alpha, ab, _p(G.bprob32_d), _p(act), gb, hb, lp, s)
+            _lib.check(rc, "asm")
 
-    def gradient(self, scaled, x, g):
-        self._eval(scaled, x, True, False, False)
-        _lib.check(_lib.lib().amigo_gather_sum(self.n, _p(self.g_ptr_d), _p(self.g_src_d), _p(self.gbuf),
-                                               _p(g), 0, _s()), "asm")
+    def gradient(self, scaled, x, g, mask=None):
+        """g for the problems of mask (all when None; the other rows keep
+        their values)."""
+        self._eval(scaled, x, True, False, False, mask)
+        act = self.active_dev(mask)
+        if act is None:
+            rc = _lib.lib().amigo_gather_sum(self.n, _p(self.g_ptr_d), _p(self.g_src_d), _p(self.gbuf),
+                                             _p(g), 0, _s())
+        else:
+            rc = _lib.lib().amigo_gather_sum_masked(self.n, _p(self.g_ptr_d), _p(self.g_src_d), _p(self.gbuf),
+                                                    _p(g), 0, _p(self.xseg64_d), self.K, _p(act), _s())
+        _lib.check(rc, "asm")
 
-    def hessian(self, scaled, x, mat):
-        self._eval(scaled, x, False, True, False)
-        _lib.check(_lib.lib().amigo_gather_sum(self.nnz, _p(self.h_ptr_d), _p(self.h_src_d), _p(self.hbuf),
-                                               _p(mat.data), 0, _s()), "asm")
+    def hessian(self, scaled, x, mat, mask=None):
+        """Hessian values of the problems of mask (all when None; the other
+        problems' entries keep their values)."""
+        self._eval(scaled, x, False, True, False, mask)
+        act = self.active_dev(mask)
+        if act is None:
+            rc = _lib.lib().amigo_gather_sum(self.nnz, _p(self.h_ptr_d), _p(self.h_src_d), _p(self.hbuf),
+                                             _p(mat.data), 0, _s())
+        else:
+            rc = _lib.lib().amigo_gather_sum_masked(self.nnz, _p(self.h_ptr_d), _p(self.h_src_d),
+                                                    _p(self.hbuf), _p(mat.data), 0, _p(self.hseg64_d),
+                                                    self.K, _p(act), _s())
+        _lib.check(rc, "asm")
 
     def lagrangian(self, scaled, x):
         """Per-problem alpha_k f_k + lambda_k^T c_k (K values, host)."""
@@ -358,8 +397,8 @@ class BatchSolver:
         self.ldl.factor(shift=self.shift)
         self.n_factor += 1
 
-    def assemble_hessian(self, scaled, x):
-        self.bp.hessian(scaled, x, self.hess)
+    def assemble_hessian(self, scaled, x, mask=None):
+        self.bp.hessian(scaled, x, self.hess, mask)
 
     def add_diagonal_and_factor(self, diag, mask=None):
         """Factor the problems of mask (all when None); the others keep
@@ -388,10 +427,27 @@ class BatchSolver:
         self.ldl.solve(px)
         self.n_solve += 1
 
-    def kkt_matvec(self, x, y):
+    def kkt_matvec(self, x, y, mask=None):
+        """y = (H + diag(shift)) x, for the rows of the problems of mask (all
+        when None; the other rows of y are not defined)."""
         rowp_d, cols_d = self.hess.pattern_device()
-        _lib.check(_lib.lib().amigo_csr_spmv(self.nrows, _p(rowp_d), _p(cols_d), _p(self.hess.data),
-                                             _p(x), _p(y), _s()), "ipm")
+        if mask is None or bool(np.all(mask)):
+            _lib.check(_lib.lib().amigo_csr_spmv(self.nrows, _p(rowp_d), _p(cols_d), _p(self.hess.data),
+                                                 _p(x), _p(y), _s()), "ipm")
+        else:
+            if getattr(self, "_row_prob", None) is None:
+                K = self.bp.K
+                self._row_prob = torch.as_tensor(np.repeat(np.arange(K, dtype=np.int32), np.diff(self.bp.xseg)),
+                                                 dtype=torch.int32, device=self.shift.device)
+                self._mv_act = torch.zeros(K, dtype=torch.int32, device=self.shift.device)
+                self._mv_last = None
+            m = np.ascontiguousarray(mask, dtype=np.int32)
+            if self._mv_last is None or not np.array_equal(m, self._mv_last):
+                self._mv_act.copy_(torch.from_numpy(m))
+                self._mv_last = m.copy()
+            _lib.check(_lib.lib().amigo_csr_spmv_masked(
+                self.nrows, _p(rowp_d), _p(cols_d), _p(self.hess.data), _p(x), _p(y),
+                _p(self._row_prob), _p(self._mv_act), _s()), "ipm")
         y.addcmul_(self.shift, x)
 
 
@@ -688,12 +744,21 @@ class BatchIPM:
         _lib.check(_lib.lib().amigo_ipm_batch_scale_rows(self.B(None), _p(self.cidx), _p(self.constr_scale),
                                                          _p(grad), 0, _s()), "ipm")
 
-    def apply_hessian_scaling(self, hess):
+    def apply_hessian_scaling(self, hess, mask=None):
         if not self.scaling_active:
             return
         rowp_d, cols_d = hess.pattern_device()
-        _lib.check(_lib.lib().amigo_csr_scale_symmetric(self.n, _p(rowp_d), _p(cols_d), _p(self.scale_vec),
-                                                        _p(hess.data), _s()), "ipm")
+        act = self.bp.active_dev(mask)
+        if act is None:
+            rc = _lib.lib().amigo_csr_scale_symmetric(self.n, _p(rowp_d), _p(cols_d), _p(self.scale_vec),
+                                                      _p(hess.data), _s())
+        else:
+            if getattr(self, "_xprob32", None) is None:
+                self._xprob32 = self.xprob.to(torch.int32)
+            rc = _lib.lib().amigo_csr_scale_symmetric_masked(self.n, _p(rowp_d), _p(cols_d),
+                                                             _p(self.scale_vec), _p(hess.data),
+                                                             _p(self._xprob32), _p(act), _s())
+        _lib.check(rc, "ipm")
 
     def unscale_multipliers(self, x):
         if self.scaling_active:
@@ -1031,7 +1096,7 @@ class BatchOptimizer:
         """grad = scaled [alpha grad f + J^T lambda; c] at x for the masked
         problems (newton_direction.py:17-27)."""
         out = self.grad if mask is None else self._gs
-        self.bp.gradient(True, x, out)
+        self.bp.gradient(True, x, out, mask)
         self.be.apply_gradient_scaling(out)
         if mask is not None:
             self.be.copy_x(self.grad, out, mask)
@@ -1185,8 +1250,9 @@ class BatchOptimizer:
     def _factorize(self, A, options):
         """Returns factorize_ok per problem (for the problems in A)."""
         be, ic, K = self.be, self.ic, self.K
-        self.solver.assemble_hessian(True, self.x)
-        be.apply_hessian_scaling(self.solver.hess)
+        # Only the problems of A are refactored and read their matrices.
+        self.solver.assemble_hessian(True, self.x, A)
+        be.apply_hessian_scaling(self.solver.hess, A)
         itol = options.get("inertia_tolerance", 0)
         max_corr = options["max_inertia_corrections"]
         diag = self.diag
@@ -1298,7 +1364,7 @@ class BatchOptimizer:
         for step in range(max_steps):
             if not R.any():
                 break
-            self.solver.kkt_matvec(self.px, Kpx)
+            self.solver.kkt_matvec(self.px, Kpx, R)
             nrm_resid, nrm_res = be.ir_residual(self.mu, self.vars, self.grad, rhs, Kpx, self.px, dzl, dzu,
                                                 self.res, ezl, ezu, R)
             denom = np.minimum(nrm_res, max_cond * nrm_rhs) + nrm_rhs

@@ -282,8 +282,12 @@ __global__ __launch_bounds__(NT) void element_kernel(int type, int E, const int*
                                                      const double* __restrict__ alpha_blk,
                                                      double* __restrict__ gbuf,
                                                      double* __restrict__ hbuf,
-                                                     double* __restrict__ lpart) {
+                                                     double* __restrict__ lpart,
+                                                     const int* __restrict__ blk_grp,
+                                                     const int* __restrict__ grp_active) {
   __shared__ double red[NT / 64];
+  // A block of a problem left out (batches) writes nothing.
+  if (blk_grp && !grp_active[blk_grp[blockIdx.x]]) return;
   const int e = blockIdx.x * NT + threadIdx.x;
   if (alpha_blk) alpha = alpha_blk[blockIdx.x];
   double Lval = 0.0;
@@ -319,11 +323,22 @@ __global__ __launch_bounds__(NT) void element_kernel(int type, int E, const int*
 }
 
 // out[i] = (accumulate ? out[i] : 0) + sum_{p in [ptr[i], ptr[i+1])} buf[src[p]]
+// With seg (nseg + 1 increasing offsets of the outputs' groups), outputs of
+// groups with seg_active[g] == 0 are skipped and keep their value.
 __global__ void gather_sum_kernel(int n, const int* __restrict__ ptr,
                                   const long long* __restrict__ src,
                                   const double* __restrict__ buf, double* __restrict__ out,
-                                  int accumulate) {
+                                  int accumulate, const long long* __restrict__ seg, int nseg,
+                                  const int* __restrict__ seg_active) {
   for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
+    if (seg) {
+      int lo = 0, hi = nseg;  // seg[lo] <= i < seg[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (seg[mid] <= i) lo = mid; else hi = mid;
+      }
+      if (!seg_active[lo]) continue;
+    }
     double s = accumulate ? out[i] : 0.0;
     for (int p = ptr[i]; p < ptr[i + 1]; p++) s += buf[src[p]];
     out[i] = s;
@@ -388,7 +403,19 @@ int amigo_elements_eval(int type, int E, const int* idx, const double* data, int
 int amigo_elements_eval_scaled(int type, int E, const int* idx, const double* data, int ndata,
                                const double* x, double alpha, const double* alpha_blk,
                                double* gbuf, double* hbuf, double* lpart, void* stream) {
+  return amigo_elements_eval_masked(type, E, idx, data, ndata, x, alpha, alpha_blk, nullptr,
+                                    nullptr, gbuf, hbuf, lpart, stream);
+}
+
+int amigo_elements_eval_masked(int type, int E, const int* idx, const double* data, int ndata,
+                               const double* x, double alpha, const double* alpha_blk,
+                               const int* blk_grp, const int* grp_active, double* gbuf,
+                               double* hbuf, double* lpart, void* stream) {
   if (E <= 0) return 0;
+  if ((blk_grp == nullptr) != (grp_active == nullptr)) {
+    g_asm_err = "blk_grp and grp_active go together";
+    return -1;
+  }
   const int nb = (E + NT - 1) / NT;
   hipStream_t s = (hipStream_t)stream;
   const bool want_h = hbuf != nullptr;
@@ -396,10 +423,12 @@ int amigo_elements_eval_scaled(int type, int E, const int* idx, const double* da
     constexpr int NC = decltype(nc)::value;
     if (want_h)
       hipLaunchKernelGGL((element_kernel<NC, true>), dim3(nb), dim3(NT), 0, s, type, E, idx,
-                         data, ndata, x, alpha, alpha_blk, gbuf, hbuf, lpart);
+                         data, ndata, x, alpha, alpha_blk, gbuf, hbuf, lpart, blk_grp,
+                         grp_active);
     else
       hipLaunchKernelGGL((element_kernel<NC, false>), dim3(nb), dim3(NT), 0, s, type, E, idx,
-                         data, ndata, x, alpha, alpha_blk, gbuf, hbuf, lpart);
+                         data, ndata, x, alpha, alpha_blk, gbuf, hbuf, lpart, blk_grp,
+                         grp_active);
   };
   switch (type) {
     case AMIGO_ELEM_LINK:
@@ -428,11 +457,21 @@ int amigo_elements_eval_scaled(int type, int E, const int* idx, const double* da
 
 int amigo_gather_sum(int n, const int* ptr, const long long* src, const double* buf,
                      double* out, int accumulate, void* stream) {
+  return amigo_gather_sum_masked(n, ptr, src, buf, out, accumulate, nullptr, 0, nullptr, stream);
+}
+
+int amigo_gather_sum_masked(int n, const int* ptr, const long long* src, const double* buf,
+                            double* out, int accumulate, const long long* seg, int nseg,
+                            const int* seg_active, void* stream) {
   if (n <= 0) return 0;
+  if ((seg == nullptr) != (seg_active == nullptr) || (seg && nseg <= 0)) {
+    g_asm_err = "seg, nseg and seg_active go together";
+    return -1;
+  }
   int nb = (n + NT - 1) / NT;
   if (nb > 4096) nb = 4096;
   hipLaunchKernelGGL(gather_sum_kernel, dim3(nb), dim3(NT), 0, (hipStream_t)stream, n, ptr, src,
-                     buf, out, accumulate);
+                     buf, out, accumulate, seg, nseg, seg_active);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err("gather_sum_kernel", e);
   return 0;

@@ -757,14 +757,19 @@ __global__ void floor_slacks_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S
 
 // ---- sparse matrix helpers -------------------------------------------------
 // y = A x for the full symmetric CSR (one wave per row).
+// With row_grp, the rows whose group is not active (grp_active[row_grp[r]]
+// == 0: problems of a batch left out) are skipped and keep their y.
 __global__ void spmv_kernel(int n, const int* __restrict__ rowp,
                             const int* __restrict__ cols,
                             const double* __restrict__ vals,
-                            const double* __restrict__ x, double* __restrict__ y) {
+                            const double* __restrict__ x, double* __restrict__ y,
+                            const int* __restrict__ row_grp,
+                            const int* __restrict__ grp_active) {
   const int lane = threadIdx.x & 63;
   const int row = (blockIdx.x * NT + threadIdx.x) >> 6;
   const int nw = (gridDim.x * NT) >> 6;
   for (int r = row; r < n; r += nw) {
+    if (row_grp && !grp_active[row_grp[r]]) continue;
     double s = 0.0;
     for (int p = rowp[r] + lane; p < rowp[r + 1]; p += 64) s += vals[p] * x[cols[p]];
 #pragma unroll
@@ -776,10 +781,13 @@ __global__ void spmv_kernel(int n, const int* __restrict__ rowp,
 // vals[p] *= sv[row] * sv[col] (interior_point_optimizer.h:573-584).
 __global__ void scale_csr_kernel(int n, const int* __restrict__ rowp,
                                  const int* __restrict__ cols,
-                                 const double* __restrict__ sv, double* __restrict__ vals) {
+                                 const double* __restrict__ sv, double* __restrict__ vals,
+                                 const int* __restrict__ row_grp,
+                                 const int* __restrict__ grp_active) {
   // One thread per row (KKT rows hold a few entries; each entry's product
   // is the same whichever thread forms it).
   for (int r = blockIdx.x * NT + threadIdx.x; r < n; r += gridDim.x * NT) {
+    if (row_grp && !grp_active[row_grp[r]]) continue;
     const double d = sv[r];
     for (int p = rowp[r]; p < rowp[r + 1]; p++) vals[p] *= d * sv[cols[p]];
   }
@@ -1159,23 +1167,37 @@ int amigo_ipm_floor_slacks(const amigo_ipm_problem* P, const amigo_ipm_state* S,
 
 int amigo_csr_spmv(int n, const int* rowp, const int* cols, const double* vals,
                    const double* x, double* y, void* stream) {
+  return amigo_csr_spmv_masked(n, rowp, cols, vals, x, y, nullptr, nullptr, stream);
+}
+
+int amigo_csr_spmv_masked(int n, const int* rowp, const int* cols, const double* vals,
+                          const double* x, double* y, const int* row_grp,
+                          const int* grp_active, void* stream) {
+  if ((row_grp == nullptr) != (grp_active == nullptr)) return -1;
   const int waves = n;
   int nb = (waves * 64 + NT - 1) / NT;
   if (nb > 8192) nb = 8192;
   if (nb < 1) nb = 1;
   hipLaunchKernelGGL(spmv_kernel, dim3(nb), dim3(NT), 0, (hipStream_t)stream, n, rowp, cols,
-                     vals, x, y);
+                     vals, x, y, row_grp, grp_active);
   LAUNCH_CHECK("spmv_kernel");
   return 0;
 }
 
 int amigo_csr_scale_symmetric(int n, const int* rowp, const int* cols,
                               const double* sv, double* vals, void* stream) {
+  return amigo_csr_scale_symmetric_masked(n, rowp, cols, sv, vals, nullptr, nullptr, stream);
+}
+
+int amigo_csr_scale_symmetric_masked(int n, const int* rowp, const int* cols,
+                                     const double* sv, double* vals, const int* row_grp,
+                                     const int* grp_active, void* stream) {
+  if ((row_grp == nullptr) != (grp_active == nullptr)) return -1;
   int nb = (n + NT - 1) / NT;
   if (nb > 65535) nb = 65535;
   if (nb < 1) nb = 1;
   hipLaunchKernelGGL(scale_csr_kernel, dim3(nb), dim3(NT), 0, (hipStream_t)stream, n, rowp,
-                     cols, sv, vals);
+                     cols, sv, vals, row_grp, grp_active);
   LAUNCH_CHECK("scale_csr_kernel");
   return 0;
 }

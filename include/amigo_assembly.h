@@ -53,6 +53,27 @@ int amigo_gather_sum(int n, const int* ptr, const long long* src,
                      const double* buf, double* out, int accumulate,
                      void* stream);
 
+/* amigo_elements_eval_scaled for the blocks of the active groups only:
+ * block b (256 elements) is evaluated when grp_active[blk_grp[b]] != 0 and
+ * otherwise writes nothing (a batch's problems still iterating; no
+ * counterpart in the reference, which evaluates one problem at a time).
+ * blk_grp: one device int per block; grp_active: device ints by group;
+ * both NULL = every block. */
+int amigo_elements_eval_masked(int type, int E, const int* idx, const double* data,
+                               int ndata, const double* x, double alpha,
+                               const double* alpha_blk, const int* blk_grp,
+                               const int* grp_active, double* gbuf, double* hbuf,
+                               double* lpart, void* stream);
+
+/* amigo_gather_sum for the outputs of the active groups only: the outputs
+ * are grouped by seg (nseg + 1 increasing device offsets, seg[0] = 0,
+ * seg[nseg] = n); outputs of groups with seg_active[g] == 0 keep their
+ * value (batches, as above).  seg = seg_active = NULL: every output. */
+int amigo_gather_sum_masked(int n, const int* ptr, const long long* src,
+                            const double* buf, double* out, int accumulate,
+                            const long long* seg, int nseg, const int* seg_active,
+                            void* stream);
+
 /* out_dev[0] = sum of part[0..nb) in order; copied to out_host if given. */
 int amigo_sum_partials(const double* part, int nb, double* out_dev,
                        double* out_host, void* stream);

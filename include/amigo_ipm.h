@@ -115,10 +115,23 @@ int amigo_ipm_floor_slacks(const amigo_ipm_problem* P, const amigo_ipm_state* S,
 /* CSRMat::mult (include/csr_matrix.h:590-600) on the full pattern. */
 int amigo_csr_spmv(int n, const int* rowp, const int* cols, const double* vals,
                    const double* x, double* y, void* stream);
+/* amigo_csr_spmv for the rows of the active groups only: row r is computed
+ * when grp_active[row_grp[r]] != 0 and otherwise keeps its y (a batch's
+ * refinement of the problems still refining; no counterpart in the
+ * reference, which solves one problem at a time).  row_grp: n device ints,
+ * grp_active: device ints indexed by group; both NULL = every row. */
+int amigo_csr_spmv_masked(int n, const int* rowp, const int* cols, const double* vals,
+                          const double* x, double* y, const int* row_grp,
+                          const int* grp_active, void* stream);
 
 /* InteriorPointOptimizer::apply_hessian_scaling (optimizer.h:573-584). */
 int amigo_csr_scale_symmetric(int n, const int* rowp, const int* cols,
                               const double* sv, double* vals, void* stream);
+/* amigo_csr_scale_symmetric for the rows of the active groups only (as
+ * amigo_csr_spmv_masked; batches, no counterpart in the reference). */
+int amigo_csr_scale_symmetric_masked(int n, const int* rowp, const int* cols,
+                                     const double* sv, double* vals, const int* row_grp,
+                                     const int* grp_active, void* stream);
 
 /* Constraint-row scaling factors of compute_nlp_scaling
  * (optimizer.h:524-540). */
