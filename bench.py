@@ -69,6 +69,8 @@ def parse():
     ap.add_argument("--serve-lockstep-requests", type=int, default=96,
                     help="requests per arrival rate of the lockstep-batching server")
     ap.add_argument("--serve-lockstep-rates", default="100,1000,5000")
+    ap.add_argument("--serve-lockstep-batches", type=int, default=4,
+                    help="lockstep batches the server runs at once")
     ap.add_argument("--timing-steps", type=int, default=1,
                     help="solves timed per kernel class with HIP events (roofline)")
     return ap.parse_args()
@@ -378,13 +380,13 @@ def lockstep_serving(args):
         seq_m = [f.result() for f in [srv.submit(sp) for sp in specs[:ns]]]
     out = {"requests_per_rate": n,
            "mix": "70% cart-pole N~U{32..1024}, 30% 2-D elliptic side~U{32..192}",
-           "form": "lockstep batches of the requests queued while earlier batches run (one problem type per batch), up to 4 batches "
-                   "at once on their own streams, mixed precision; each result returned when its own "
-                   "problem finishes",
+           "form": "lockstep batches of the requests queued while earlier batches run (one problem type per batch), "
+                   f"up to {args.serve_lockstep_batches} batches at once on their own streams, mixed precision; "
+                   "each result returned when its own problem finishes",
            "sequential_fp64_sample": ns, "sequential_fp64_solves_per_s": seq_rate, "rates": []}
     same = True
     for rate in [float(r) for r in args.serve_lockstep_rates.split(",") if r]:
-        with BatchSolveServer(max_batch=512, max_batches=4) as srv:
+        with BatchSolveServer(max_batch=512, max_batches=args.serve_lockstep_batches) as srv:
             st = run_poisson(srv, specs, rate=rate, seed=args.seed + 12)
         sizes = [b for b, _ in srv.batches]
         mism = [(i, specs[i]["type"], bool(st["results"][i]["batched"]),
@@ -404,6 +406,12 @@ def lockstep_serving(args):
             "speedup_vs_sequential_fp64": st["solves_per_s"] / seq_rate,
             "sample_max_abs_diff_vs_fp64": dmax,
             "sample_mismatches_vs_same_precision": mism,
+            # Median parts of a request's latency: waiting for its batch to
+            # start, building the batch (problems, union plan, buffers), and
+            # solving until its own problem finished.
+            "p50_queue_ms": 1e3 * float(np.median([r["queue_s"] for r in st["results"]])),
+            "p50_build_ms": 1e3 * float(np.median([r["build_s"] for r in st["results"]])),
+            "p50_solve_ms": 1e3 * float(np.median([r["solve_s"] for r in st["results"]])),
         })
         print(f"[bench] lockstep serving at {rate:.0f} req/s: {st['solves_per_s']:.1f} solves/s, "
               f"p50 {st['p50_ms']:.0f} ms, p99 {st['p99_ms']:.0f} ms, {len(sizes)} batches",
