@@ -47,6 +47,7 @@ def lib():
         sig = {
             "ref_order": (_ci, [_ci, _ci, _P(_ci), _P(_ci), _P(_ci), _P(_ci)]),
             "ref_block_amd": (_ci, [_ci, _P(_ci), _P(_ci), _ci, _P(_ci), _P(_ci)]),
+            "ref_element_csr": (_ci, [_ci, _ci, _P(_ci), _P(_ci), _P(_ci), _P(_ci), _P(_ci), _P(_ci), _ci]),
             "ref_ldl_create": (ctypes.c_void_p, [_ci, _P(_ci), _P(_ci), _ci, _cd, _cd, _cd, _ci]),
             "ref_ldl_destroy": (None, [ctypes.c_void_p]),
             "ref_ldl_num_nodes": (_ci, [ctypes.c_void_p]),
@@ -90,6 +91,36 @@ def order(rowp, cols, kind="AMD"):
     iperm = np.zeros(n, np.int32)
     lib().ref_order(ORDER[kind], n, _ip(rowp), _ip(cols), _ip(perm), _ip(iperm))
     return perm, iperm
+
+
+def element_csr(nrows, arrays, num_cons):
+    """The reference's element-graph CSR (create_csr_from_element_conn as
+    reorder_model calls it) of element arrays (E_g x npe_g node indices)
+    with num_cons[g] constraint nodes last."""
+    ne = np.array([a.shape[0] for a in arrays], np.int32)
+    npe = np.array([a.shape[1] for a in arrays], np.int32)
+    nc = np.ascontiguousarray(num_cons, np.int32)
+    conn = np.ascontiguousarray(np.concatenate([np.ascontiguousarray(a, np.int32).ravel() for a in arrays]))
+    rowp = np.zeros(nrows + 1, np.int32)
+    cols = np.zeros(1, np.int32)
+    nnz = lib().ref_element_csr(nrows, len(arrays), _ip(ne), _ip(npe), _ip(nc), _ip(conn), _ip(rowp), _ip(cols), 0)
+    cols = np.zeros(max(nnz, 1), np.int32)
+    lib().ref_element_csr(nrows, len(arrays), _ip(ne), _ip(npe), _ip(nc), _ip(conn), _ip(rowp), _ip(cols), nnz)
+    return rowp, cols[:nnz]
+
+
+def block_amd(rowp, cols, mult=()):
+    """Raw BlockAMD::amd on the given pattern (multipliers last)."""
+    rowp = np.ascontiguousarray(rowp, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    m = np.ascontiguousarray(mult, np.int32)
+    n = len(rowp) - 1
+    perm = np.zeros(n, np.int32)
+    rc = lib().ref_block_amd(n, _ip(rowp), _ip(cols), len(m), _ip(m if m.size else np.zeros(1, np.int32)),
+                             _ip(perm))
+    if rc != 0:
+        raise RuntimeError("reference BlockAMD failed")
+    return perm
 
 
 class RefLDL:

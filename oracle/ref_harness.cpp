@@ -117,6 +117,37 @@ int ref_block_amd(int n, const int* rowp, const int* cols, int nmult,
   return st == BlockAMD::AMDStatus::SUCCESS ? 0 : 1;
 }
 
+// OrderingUtils::create_csr_from_element_conn (ordering_utils.h:374-470) as
+// reorder_model calls it (amigo/amigo.cpp:228-264: no diagonal, sorted rows,
+// no linear elements): element groups one after another in conn, group g
+// with ne[g] elements of npe[g] nodes whose last ncon[g] are constraint
+// nodes.  Returns nnz; fills rowp (nrows + 1) and, when nnz <= cap, cols.
+int ref_element_csr(int nrows, int ngroups, const int* ne, const int* npe, const int* ncon,
+                    const int* conn, int* rowp_out, int* cols_out, int cap) {
+  std::vector<int> start(ngroups + 1, 0);
+  std::vector<size_t> off(ngroups + 1, 0);
+  for (int g = 0; g < ngroups; g++) {
+    start[g + 1] = start[g] + ne[g];
+    off[g + 1] = off[g] + (size_t)ne[g] * npe[g];
+  }
+  auto nodes = [&](int e, const int** ptr, int* nc = nullptr, bool* lin = nullptr) {
+    const int g = int(std::upper_bound(start.begin(), start.end(), e) - start.begin()) - 1;
+    *ptr = conn + off[g] + (size_t)(e - start[g]) * npe[g];
+    if (nc) *nc = ncon[g];
+    if (lin) *lin = false;
+    return npe[g];
+  };
+  int *rp = nullptr, *cl = nullptr;
+  OrderingUtils::create_csr_from_element_conn(nrows, nrows, start[ngroups], nodes, false, true,
+                                              &rp, &cl);
+  const int nnz = rp[nrows];
+  std::copy(rp, rp + nrows + 1, rowp_out);
+  if (nnz <= cap) std::copy(cl, cl + nnz, cols_out);
+  delete[] rp;
+  delete[] cl;
+  return nnz;
+}
+
 void* ref_ldl_create(int n, const int* rowp, const int* cols, int cholesky,
                      double ustab, double pivot_tol, double delay_growth,
                      int order) {
