@@ -205,6 +205,7 @@ _SIGS.update(
         "amigo_ipm_reduce": (_I, [_I, _PP, _PS, _D, _V, _V, _V, _V, _c_dbl_p, _V]),
         "amigo_ipm_reset_bound_multipliers": (_I, [_PP, _PS, _D, _D, _V]),
         "amigo_ipm_dual_residual_vector": (_I, [_PP, _PS, _V, _V, _I, _V]),
+        "amigo_ipm_affine_start_point": (_I, [_PP, _PS, _D, _V, _V, _V, _V, _V]),
         "amigo_ipm_initialize": (_I, [_V, _V, _I, _V, _D, _D, _D, _D, _V, _V, _PS, _I, _I, _V]),
         "amigo_ipm_floor_slacks": (_I, [_PP, _PS, _V, _V, _D, _D, _D, _V, _c_dbl_p, _V]),
         "amigo_csr_spmv": (_I, [_I, _V, _V, _V, _V, _V, _V]),

@@ -638,6 +638,24 @@ __global__ void reset_multipliers_kernel(BT B, amigo_ipm_problem P, amigo_ipm_st
   }
 }
 
+// z + dz floored at beta_min where the bound is finite, written to the
+// output duals (backend.h:480-489); other entries are left as they are.
+__global__ void affine_start_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S,
+                                    double beta_min, const double* __restrict__ dzl,
+                                    const double* __restrict__ dzu, double* __restrict__ zl_out,
+                                    double* __restrict__ zu_out) {
+  const View V = view(B, P, S);
+  if (V.skip) return;
+  dzl += V.p0;
+  dzu += V.p0;
+  zl_out += V.p0;
+  zu_out += V.p0;
+  for (int i = V.bx * NT + threadIdx.x; i < P.n_primal; i += V.gx * NT) {
+    if (finite_bound(P.lbx[i])) zl_out[i] = fmax(S.zl[i] + dzl[i], beta_min);
+    if (finite_bound(P.ubx[i])) zu_out[i] = fmax(S.zu[i] + dzu[i], beta_min);
+  }
+}
+
 // r_d = grad - zl + zu on primals, 0 elsewhere (backend.h:566-574).
 __global__ void dual_residual_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S,
                                      const double* __restrict__ grad,
@@ -753,6 +771,25 @@ __global__ void floor_slacks_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S
   }
   block_reduce<1>(v, ops, red);
   if (threadIdx.x == 0) ws[V.slot] = v[0];
+}
+
+// After slack flooring moved any slack of a problem, every bounded primal of
+// that problem is set from its slack: x = lb + sl, then x = ub - su where
+// the upper bound is finite (bound_safeguards.py:68-80).  count[k] is the
+// problem's number of adjusted slacks (device).
+__global__ void snap_primals_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S,
+                                    const double* __restrict__ lbx0,
+                                    const double* __restrict__ ubx0,
+                                    const double* __restrict__ count) {
+  const View V = view(B, P, S);
+  if (V.skip || !(count[V.k] > 0.0)) return;
+  lbx0 += V.p0;
+  ubx0 += V.p0;
+  for (int i = V.bx * NT + threadIdx.x; i < P.n_primal; i += V.gx * NT) {
+    const int idx = P.primal_indices[i];
+    if (finite_bound(lbx0[i])) S.xlam[idx] = lbx0[i] + S.sl[i];
+    if (finite_bound(ubx0[i])) S.xlam[idx] = ubx0[i] - S.su[i];
+  }
 }
 
 // ---- sparse matrix helpers -------------------------------------------------
@@ -1128,6 +1165,15 @@ int amigo_ipm_reset_bound_multipliers(const amigo_ipm_problem* P,
   return 0;
 }
 
+int amigo_ipm_affine_start_point(const amigo_ipm_problem* P, const amigo_ipm_state* S,
+                                 double beta_min, const double* dzl, const double* dzu,
+                                 double* zl_out, double* zu_out, void* stream) {
+  hipLaunchKernelGGL(affine_start_kernel, dim3(grid_for(P->n_primal)), dim3(NT), 0,
+                     (hipStream_t)stream, BT{}, *P, *S, beta_min, dzl, dzu, zl_out, zu_out);
+  LAUNCH_CHECK("affine_start_kernel");
+  return 0;
+}
+
 int amigo_ipm_dual_residual_vector(const amigo_ipm_problem* P,
                                    const amigo_ipm_state* S, const double* grad,
                                    double* out, int n, void* stream) {
@@ -1162,7 +1208,13 @@ int amigo_ipm_floor_slacks(const amigo_ipm_problem* P, const amigo_ipm_state* S,
                      mu, nullptr, s_min, nullptr, slack_move, ws);
   LAUNCH_CHECK("floor_slacks_kernel");
   const int ops[1] = {SUM};
-  return finish_and_copy(1, ops, ws, nb, out1, s);
+  const int rc = finish_and_copy(1, ops, ws, nb, out1, s);
+  if (rc) return rc;
+  // finish_and_copy leaves the count at ws[MAXB * 4] on the device.
+  hipLaunchKernelGGL(snap_primals_kernel, dim3(nb), dim3(NT), 0, s, BT{}, *P, *S, lbx0, ubx0,
+                     ws + (size_t)MAXB * 4);
+  LAUNCH_CHECK("snap_primals_kernel");
+  return 0;
 }
 
 int amigo_csr_spmv(int n, const int* rowp, const int* cols, const double* vals,
@@ -1447,7 +1499,12 @@ int amigo_ipm_batch_floor_slacks(const amigo_ipm_batch* B, const amigo_ipm_probl
                      ubx0, 0.0, mu, 0.0, s_min, slack_move, ws);
   LAUNCH_CHECK("floor_slacks_kernel");
   const int ops[1] = {SUM};
-  return batch_finish(B, bt, 1, ops, ws, out_dev, out1, s);
+  const int rc = batch_finish(B, bt, 1, ops, ws, out_dev, out1, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(snap_primals_kernel, dim3(B->nblk_primal), dim3(NT), 0, s, bt, *P, *S, lbx0,
+                     ubx0, out_dev);
+  LAUNCH_CHECK("snap_primals_kernel");
+  return 0;
 }
 
 int amigo_ipm_batch_scale_rows(const amigo_ipm_batch* B, const int* cidx, const double* dc,

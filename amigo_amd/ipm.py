@@ -280,10 +280,13 @@ class InteriorPointOptimizer:
 
     def compute_affine_start_point(self, beta_min, vars_, upd, dst):
         """ipm::compute_affine_start_point (backend.h:480-489)."""
-        fl = torch.isfinite(self.lbx)
-        fu = torch.isfinite(self.ubx)
-        dst.zl.copy_(torch.where(fl, torch.clamp(vars_.zl + upd.zl, min=beta_min), dst.zl))
-        dst.zu.copy_(torch.where(fu, torch.clamp(vars_.zu + upd.zu, min=beta_min), dst.zu))
+        _lib.check(
+            _lib.lib().amigo_ipm_affine_start_point(
+                self.info, vars_.state(), float(beta_min), _lib.dptr(upd.zl), _lib.dptr(upd.zu),
+                _lib.dptr(dst.zl), _lib.dptr(dst.zu), _s(),
+            ),
+            "ipm",
+        )
 
     def compute_dual_residual_vector(self, vars_, grad, out):
         _lib.check(

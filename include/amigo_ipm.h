@@ -92,6 +92,14 @@ int amigo_ipm_reset_bound_multipliers(const amigo_ipm_problem* P,
                                       const amigo_ipm_state* S, double mu,
                                       double kappa, void* stream);
 
+/* ipm::compute_affine_start_point (backend.h:480-489): zl_out = max(zl +
+ * dzl, beta_min) and zu_out likewise where the bound is finite; the other
+ * entries of zl_out / zu_out are left as they are (in place when they are
+ * S's duals). */
+int amigo_ipm_affine_start_point(const amigo_ipm_problem* P, const amigo_ipm_state* S,
+                                 double beta_min, const double* dzl, const double* dzu,
+                                 double* zl_out, double* zu_out, void* stream);
+
 /* ipm::compute_dual_residual_vector (backend.h:566-574). */
 int amigo_ipm_dual_residual_vector(const amigo_ipm_problem* P,
                                    const amigo_ipm_state* S, const double* grad,

@@ -293,3 +293,240 @@ def di_discrete_optimum(N, T=1.0):
     sol = np.linalg.solve(K, np.concatenate([np.zeros(nz), rhs]))
     z, lam = sol[:nz], sol[nz:]
     return z[iq], z[iv], z[iu], lam[:N], lam[N:2 * N]
+
+
+# ---------------------------------------------------------------------------
+# Further reductions and vector kernels of interior_point_backend.h.
+
+def kkt_error_sq(P, S, grad):
+    """compute_kkt_error_sq (backend.h:530-549): sums of squares of the
+    stationarity rows, the constraint residuals and the complementarity
+    products (not their deviation from mu)."""
+    _, zl, zu, sl, su = S
+    rd = grad[P.pidx] - zl + zu
+    rp = grad[P.cidx] - P.lbh
+    cl = (sl * zl)[P.fl]
+    cu = (su * zu)[P.fu]
+    return float(rd @ rd), float(rp @ rp), float(cl @ cl + cu @ cu)
+
+
+def max_comp_deviation(mu, P, S):
+    """compute_max_comp_deviation (backend.h:417-427), starting at 0."""
+    _, zl, zu, sl, su = S
+    d = np.concatenate([np.abs((sl * zl)[P.fl] - mu), np.abs((su * zu)[P.fu] - mu)])
+    return float(np.max(d, initial=0.0))
+
+
+def complementarity_sq(mu, P, S):
+    """compute_complementarity_sq (backend.h:430-444)."""
+    _, zl, zu, sl, su = S
+    d = np.concatenate([(sl * zl)[P.fl] - mu, (su * zu)[P.fu] - mu])
+    return float(d @ d)
+
+
+def barrier_dphi(mu, P, S, grad, px):
+    """compute_barrier_dphi (backend.h:513-524): grad.dx - mu dx/sl +
+    mu dx/su over the primals."""
+    _, _, _, sl, su = S
+    dx = px[P.pidx]
+    d = np.sum(grad[P.pidx] * dx)
+    d -= mu * np.sum((dx / np.where(P.fl, sl, 1.0))[P.fl])
+    d += mu * np.sum((dx / np.where(P.fu, su, 1.0))[P.fu])
+    return float(d)
+
+
+def affine_start_point(beta_min, P, S, dzl, dzu):
+    """compute_affine_start_point (backend.h:480-489): z + dz floored at
+    beta_min where the bound is finite; other entries unchanged."""
+    _, zl, zu, _, _ = S
+    zl2 = np.where(P.fl, np.maximum(zl + dzl, beta_min), zl)
+    zu2 = np.where(P.fu, np.maximum(zu + dzu, beta_min), zu)
+    return zl2, zu2
+
+
+def dual_residual_vector(P, S, grad):
+    """compute_dual_residual_vector (backend.h:566-574)."""
+    _, zl, zu, _, _ = S
+    out = np.zeros(P.n)
+    out[P.pidx] = grad[P.pidx] - zl + zu
+    return out
+
+
+def abs_sums(P, S):
+    """The sums behind the optimality scaling of
+    amigo/algorithm/optimality_scaling.py:19-48:
+    [sum |zl| + |zu|, sum |y|, number of finite bounds, number of
+    constraints]."""
+    x, zl, zu, _, _ = S
+    return (float(np.sum(np.abs(zl)) + np.sum(np.abs(zu))), float(np.sum(np.abs(x[P.cidx]))),
+            float(np.sum(P.fl) + np.sum(P.fu)), float(len(P.cidx)))
+
+
+def optimality_scaling(P, S, s_max=100.0):
+    """(s_d, s_c) of optimality_scaling.py:19-48."""
+    z, y, nb, nc = abs_sums(P, S)
+    s_c = 1.0 if nb == 0 else max(s_max, z / nb) / s_max
+    s_d = 1.0 if nb + nc == 0 else max(s_max, (y + z) / (nb + nc)) / s_max
+    return s_d, s_c
+
+
+def floor_slacks(mu, P, S, lbx0, ubx0, slack_move=1.81898940354586e-12):
+    """Slack flooring of amigo/algorithm/bound_safeguards.py:16-87.  Slacks
+    below s_min = eps min(1, mu) become min(max(mu/z, s_min), slack_move
+    max(1, |bound|) + max(s, 0)); when any slack moved, every bounded
+    primal is set from its slack (x = lb + sl, then x = ub - su), as the
+    reference's consistency loops do.  Returns the new state and the count
+    of adjusted slacks."""
+    x, zl, zu, sl, su = (a.copy() for a in S)
+    s_min = np.finfo(float).eps * min(1.0, mu)
+    if s_min == 0.0:
+        s_min = np.finfo(float).tiny
+    tiny = np.finfo(float).tiny
+    fl, fu = np.isfinite(lbx0), np.isfinite(ubx0)
+    adjusted = 0
+    for s, z, b, f in ((sl, zl, lbx0, fl), (su, zu, ubx0, fu)):
+        small = f & (s < s_min)
+        if small.any():
+            g = np.maximum(s[small], 0.0)
+            target = np.maximum(mu / np.maximum(z[small], tiny), s_min)
+            cap = slack_move * np.maximum(1.0, np.abs(b[small])) + g
+            s[small] = np.minimum(target, cap)
+            adjusted += int(small.sum())
+    if adjusted:
+        xp = x[P.pidx]
+        xp = np.where(fl, lbx0 + sl, xp)
+        xp = np.where(fu, ubx0 - su, xp)
+        x[P.pidx] = xp
+    return (x, zl, zu, sl, su), adjusted
+
+
+# ---------------------------------------------------------------------------
+# NLP scaling (include/interior_point_optimizer.h:502-584).
+
+def nlp_scaling(P, grad, rowp, vals, max_gradient=100.0, min_value=1e-8):
+    """compute_nlp_scaling (optimizer.h:502-563): the objective scale from
+    the inf-norm of the primal gradient, and per constraint row j
+    dc_j = min(1, max_gradient / |row_j|_inf) floored at min_value, from
+    the KKT matrix assembled at lambda = 0.  Returns (obj_scale, dc, lbh
+    scaled by dc, the per-variable scale vector)."""
+    g = np.abs(grad[P.pidx])
+    gmax = float(np.max(g, initial=0.0))
+    obj = max_gradient / gmax if gmax > max_gradient else 1.0
+    obj = max(obj, min_value)
+    dc = np.ones(len(P.cidx))
+    for j, r in enumerate(P.cidx):
+        row = np.abs(vals[rowp[r]:rowp[r + 1]])
+        m = float(np.max(row, initial=0.0))
+        d = max_gradient / m if m > max_gradient else 1.0
+        dc[j] = max(d, min_value)
+    lbh = np.where(dc != 1.0, P.lbh * dc, P.lbh)
+    sv = np.ones(P.n)
+    sv[P.cidx] = dc
+    return obj, dc, lbh, sv
+
+
+def hessian_scaling(rowp, cols, vals, sv):
+    """apply_hessian_scaling (optimizer.h:573-584): a_ij *= s_i s_j."""
+    out = vals.copy()
+    for i in range(len(rowp) - 1):
+        for p in range(rowp[i], rowp[i + 1]):
+            out[p] *= sv[i] * sv[cols[p]]
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Iterative refinement on the unreduced system
+# (amigo/algorithm/solvers/linear_solver.py:109-264).
+
+IR_RATIO_MAX = 1e-10
+IR_IMPROVEMENT = 0.999999999
+IR_MIN_STEPS = 1
+IR_MAX_COND = 1e6
+
+
+def ir_blocks(mu, P, S, grad, rhs_saved, Kpx, px, dzl, dzu):
+    """Residuals of the 8-block system at (px, dzl, dzu), their inf-norm,
+    the inf-norm of the iterate, and the condensed correction right-hand
+    side (linear_solver.py:203-245).  Kpx = (K + Sigma) px with the matrix
+    that was factored."""
+    _, zl, zu, sl, su = S
+    fl, fu = P.fl, P.fu
+    gl = np.where(fl, sl, 1.0)
+    gu = np.where(fu, su, 1.0)
+    sigma = np.where(fl, zl / gl, 0.0) + np.where(fu, zu / gu, 0.0)
+    dx = px[P.pidx]
+    rhs_x = -(grad[P.pidx] - zl + zu)
+    e_x = rhs_x - Kpx[P.pidx] + sigma * dx + dzl - dzu
+    e_c = rhs_saved[P.cidx] - Kpx[P.cidx]
+    e_zl = np.where(fl, (mu - gl * zl) - zl * dx - gl * dzl, 0.0)
+    e_zu = np.where(fu, (mu - gu * zu) + zu * dx - gu * dzu, 0.0)
+    nrm_resid = max(np.max(np.abs(a), initial=0.0) for a in (e_x, e_c, e_zl, e_zu))
+    nrm_sol = max(np.max(np.abs(a), initial=0.0) for a in (px, dzl, dzu))
+    econd = np.zeros(P.n)
+    econd[P.pidx] = e_x + np.where(fl, e_zl / gl, 0.0) - np.where(fu, e_zu / gu, 0.0)
+    econd[P.cidx] = e_c
+    return nrm_resid, nrm_sol, econd, e_zl, e_zu
+
+
+def ir_rhs_norm(mu, P, S, grad, rhs_saved):
+    """Inf-norm of the 8-block right-hand side (linear_solver.py:187-198)."""
+    _, zl, zu, sl, su = S
+    a = np.abs(grad[P.pidx] - zl + zu)
+    b = np.abs(rhs_saved[P.cidx])
+    c = np.abs(mu - sl * zl)[P.fl]
+    d = np.abs(mu - su * zu)[P.fu]
+    return max(np.max(x, initial=0.0) for x in (a, b, c, d))
+
+
+def ir_initial_dz(mu, P, S, px):
+    """Bound-dual steps by back-substitution (linear_solver.py:177-179)."""
+    _, zl, zu, sl, su = S
+    dx = px[P.pidx]
+    gl = np.where(P.fl, sl, 1.0)
+    gu = np.where(P.fu, su, 1.0)
+    dzl = np.where(P.fl, (mu - gl * zl - zl * dx) / gl, 0.0)
+    dzu = np.where(P.fu, (mu - gu * zu + zu * dx) / gu, 0.0)
+    return dzl, dzu
+
+
+def ir_stop(step, ratio, ratio_old):
+    """The stopping rule of linear_solver.py:226-236: converged (ratio below
+    1e-10 after the minimum number of steps) or stalled (ratio above
+    0.999999999 times the previous one after the minimum steps)."""
+    if step >= IR_MIN_STEPS and ratio < IR_RATIO_MAX:
+        return "converged"
+    if step >= IR_MIN_STEPS and ratio > IR_IMPROVEMENT * ratio_old:
+        return "stalled"
+    return None
+
+
+def iterative_refinement(mu, P, S, grad, rhs_saved, px, kkt_matvec, solve, max_steps=10):
+    """The refinement loop (linear_solver.py:109-264) with the caller's
+    operator y = (K + Sigma) x and solve.  Returns (px, dzl, dzu, steps,
+    final ratio, reason)."""
+    px = px.copy()
+    _, zl, zu, sl, su = S
+    dzl, dzu = ir_initial_dz(mu, P, S, px)
+    nrm_rhs = ir_rhs_norm(mu, P, S, grad, rhs_saved)
+    ratio_old = 1e30
+    ratio = 0.0
+    steps = 0
+    reason = "max_steps"
+    gl = np.where(P.fl, sl, 1.0)
+    gu = np.where(P.fu, su, 1.0)
+    for step in range(max_steps):
+        nrm_resid, nrm_sol, econd, e_zl, e_zu = ir_blocks(mu, P, S, grad, rhs_saved, kkt_matvec(px), px, dzl, dzu)
+        denom = min(nrm_sol, IR_MAX_COND * nrm_rhs) + nrm_rhs
+        ratio = nrm_resid / denom if denom > 0 else 0.0
+        why = ir_stop(step, ratio, ratio_old)
+        if why is not None:
+            reason = why
+            break
+        ratio_old = ratio
+        corr = solve(econd)
+        dc = corr[P.pidx]
+        dzl = dzl + np.where(P.fl, (e_zl - zl * dc) / gl, 0.0)
+        dzu = dzu + np.where(P.fu, (e_zu + zu * dc) / gu, 0.0)
+        px = px + corr
+        steps += 1
+    return px, dzl, dzu, steps, ratio, reason

@@ -3,6 +3,8 @@ device InteriorPointOptimizer) against the oracle restatement
 (oracle/ipm.py) on seeded random problems: element-wise results to FP64
 rounding, reductions to 1e-13 relative, argmin indices exactly."""
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -132,3 +134,160 @@ def test_reset_bound_multipliers():
     opt.reset_bound_multipliers(mu, kappa, v)
     zl, zu = O.reset_bound_multipliers(mu, kappa, P, S)
     assert np.allclose(_h(v.zl), zl, rtol=1e-15, atol=0) and np.allclose(_h(v.zu), zu, rtol=1e-15, atol=0)
+
+
+def test_initialize_relaxes_and_projects_like_the_oracle():
+    rng, P, S, opt, v = _setup(6)
+    x0 = rng.normal(scale=4.0, size=P.n)
+    v.x.copy_(_d(x0))
+    opt.relax_bounds(1e-8, 1e-4)
+    lb, ub = O.relax_bounds(P, 1e-8, 1e-4)
+    assert np.array_equal(_h(opt.lbx), lb) and np.array_equal(_h(opt.ubx), ub)
+    opt.initialize_multipliers_and_slacks(1.0, None, v)
+    Pr = O.Info(~np.isin(np.arange(P.n), P.pidx), np.zeros(P.n), np.zeros(P.n))
+    Pr.lbx, Pr.ubx, Pr.lbh = lb, ub, P.lbh
+    exp = O.project_and_init(Pr, x0)
+    # to FP64 rounding: the device contracts a*b+c into one fused operation
+    for a, b in zip((v.x, v.zl, v.zu, v.sl, v.su), exp):
+        assert np.allclose(_h(a), b, rtol=1e-15, atol=1e-15)
+
+
+@pytest.mark.parametrize("seed", [7, 8])
+def test_remaining_reductions(seed):
+    rng, P, S, opt, v = _setup(seed)
+    mu = 0.13
+    grad = rng.normal(size=P.n)
+    px = rng.normal(size=P.n)
+    assert opt.compute_kkt_error(v, _d(grad)) == pytest.approx(O.kkt_error_sq(P, S, grad), rel=1e-13)
+    assert np.isclose(opt.compute_max_comp_deviation(v, mu), O.max_comp_deviation(mu, P, S), rtol=1e-15)
+    assert np.isclose(opt.compute_complementarity_sq(v), O.complementarity_sq(0.0, P, S), rtol=1e-13)
+    assert np.isclose(opt.compute_barrier_dphi_direct(mu, v, _d(grad), _d(px)),
+                      O.barrier_dphi(mu, P, S, grad, px), rtol=1e-12)
+    got = opt.abs_sums(v)
+    exp = O.abs_sums(P, S)
+    assert np.allclose(got, exp, rtol=1e-13, atol=0)
+
+
+def test_affine_start_point_and_dual_residual():
+    rng, P, S, opt, v = _setup(9)
+    upd = opt.create_opt_vector()
+    dzl = rng.normal(size=len(P.pidx))
+    dzu = rng.normal(size=len(P.pidx))
+    upd.zl.copy_(_d(dzl))
+    upd.zu.copy_(_d(dzu))
+    opt.compute_affine_start_point(0.4, v, upd, v)
+    zl, zu = O.affine_start_point(0.4, P, S, dzl, dzu)
+    assert np.array_equal(_h(v.zl), zl) and np.array_equal(_h(v.zu), zu)
+    grad = rng.normal(size=P.n)
+    out = torch.full((P.n,), 7.0, dtype=torch.float64, device="cuda")
+    opt.compute_dual_residual_vector(v, _d(grad), out)
+    S2 = (S[0], zl, zu, S[3], S[4])
+    assert np.array_equal(_h(out), O.dual_residual_vector(P, S2, grad))
+
+
+def test_floor_slacks_matches_the_oracle():
+    rng, P, S, opt, v = _setup(10)
+    x, zl, zu, sl, su = (a.copy() for a in S)
+    il = np.nonzero(P.fl)[0]
+    iu = np.nonzero(P.fu)[0]
+    sl[il[:3]] = [1e-30, -1e-20, 0.0]
+    su[iu[:2]] = [1e-25, 0.0]
+    for name, a in zip(("x", "zl", "zu", "sl", "su"), (x, zl, zu, sl, su)):
+        getattr(v, name).copy_(_d(a))
+    mu = 1e-3
+    s_min = np.finfo(float).eps * min(1.0, mu)
+    cnt = opt.floor_slacks(v, mu, s_min, 1.81898940354586e-12)
+    exp, cnt_o = O.floor_slacks(mu, P, (x, zl, zu, sl, su), P.lbx, P.ubx)
+    assert cnt == cnt_o == 5
+    for a, b in zip((v.x, v.zl, v.zu, v.sl, v.su), exp):
+        assert np.allclose(_h(a), b, rtol=1e-15, atol=1e-300)
+
+
+def test_nlp_scaling_kernels_match_the_oracle():
+    import scipy.sparse as sp
+
+    from amigo_amd import _lib
+    from amigo_amd.ipm import _s
+
+    rng, P, S, opt, v = _setup(11, n_p=600, n_c=250)
+    n = P.n
+    A = sp.random(n, n, density=0.01, random_state=3, format="csr") * 50.0
+    A = A + A.T + sp.eye(n)
+    A = A.tocsr()
+    A.sort_indices()
+    rows = P.cidx[:20]
+    D = sp.diags(np.where(np.isin(np.arange(n), rows), 1e4, 1.0))
+    A = (D @ A @ D).tocsr()
+    A.sort_indices()
+    grad = rng.normal(size=n) * 30
+    grad[P.pidx[0]] = -700.0
+    obj, dc, lbh, sv = O.nlp_scaling(P, grad, A.indptr, A.data)
+    rowp = torch.tensor(A.indptr.astype(np.int32), device="cuda")
+    cols = torch.tensor(A.indices.astype(np.int32), device="cuda")
+    vals = _d(A.data)
+    dcd = torch.ones(len(P.cidx), dtype=torch.float64, device="cuda")
+    _lib.check(_lib.lib().amigo_ipm_constraint_scaling(
+        len(P.cidx), _lib.dptr(opt.cidx), _lib.dptr(rowp), _lib.dptr(vals), 100.0, 1e-8,
+        _lib.dptr(dcd), _s()), "ipm")
+    assert np.array_equal(_h(dcd), dc) and np.any(dc < 1.0)
+    svd = _d(sv)
+    _lib.check(_lib.lib().amigo_csr_scale_symmetric(
+        n, _lib.dptr(rowp), _lib.dptr(cols), _lib.dptr(svd), _lib.dptr(vals), _s()), "ipm")
+    assert np.array_equal(_h(vals), O.hessian_scaling(A.indptr, A.indices, A.data, sv))
+    g = _d(grad)
+    _lib.check(_lib.lib().amigo_ipm_scale_rows(len(P.cidx), _lib.dptr(opt.cidx), _lib.dptr(dcd),
+                                               _lib.dptr(g), 0, _s()), "ipm")
+    exp = grad.copy()
+    exp[P.cidx] *= dc
+    assert np.array_equal(_h(g), exp)
+
+
+def test_refinement_kernels_match_the_oracle():
+    from amigo_amd import _lib
+    from amigo_amd.ipm import _s
+
+    rng, P, S, opt, v = _setup(12)
+    mu = 0.07
+    grad = rng.normal(size=P.n)
+    rhs = rng.normal(size=P.n)
+    px = rng.normal(size=P.n)
+    Kpx = rng.normal(size=P.n)
+    L = _lib.lib()
+    # device copies held for the whole test (a temporary's memory could be
+    # reused by the next one before the kernels read it)
+    grad_d, rhs_d, px_d, Kpx_d = _d(grad), _d(rhs), _d(px), _d(Kpx)
+    npr = len(P.pidx)
+    dzl = torch.zeros(npr, dtype=torch.float64, device="cuda")
+    dzu = torch.zeros_like(dzl)
+    out = (ctypes.c_double * 4)()
+    _lib.check(L.amigo_ipm_ir_init(opt.info, v.state(), mu, _lib.dptr(grad_d), _lib.dptr(rhs_d),
+                                   _lib.dptr(px_d), _lib.dptr(dzl), _lib.dptr(dzu), _lib.dptr(opt.ws),
+                                   out, _s()), "ipm")
+    dzl_o, dzu_o = O.ir_initial_dz(mu, P, S, px)
+    # fused multiply-adds on the device; cancellation near zero
+    assert np.allclose(_h(dzl), dzl_o, rtol=1e-13, atol=1e-14) and np.allclose(_h(dzu), dzu_o, rtol=1e-13, atol=1e-14)
+    assert np.isclose(out[0], O.ir_rhs_norm(mu, P, S, grad, rhs), rtol=1e-15)
+    econd = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+    ezl = torch.zeros_like(dzl)
+    ezu = torch.zeros_like(dzl)
+    _lib.check(L.amigo_ipm_ir_residual(opt.info, v.state(), mu, _lib.dptr(grad_d), _lib.dptr(rhs_d),
+                                       _lib.dptr(Kpx_d), _lib.dptr(px_d), _lib.dptr(dzl), _lib.dptr(dzu),
+                                       _lib.dptr(econd), _lib.dptr(ezl), _lib.dptr(ezu), _lib.dptr(opt.ws),
+                                       out, _s()), "ipm")
+    nr, ns, ec, ezl_o, ezu_o = O.ir_blocks(mu, P, S, grad, rhs, Kpx, px, dzl_o, dzu_o)
+    assert np.isclose(out[0], nr, rtol=1e-13) and np.isclose(out[1], ns, rtol=1e-13)
+    assert np.allclose(_h(econd), ec, rtol=1e-13, atol=1e-14)
+    assert np.allclose(_h(ezl), ezl_o, rtol=1e-13, atol=1e-14)
+    corr = rng.normal(size=P.n)
+    corr_d = _d(corr)
+    pxd = _d(px)
+    _lib.check(L.amigo_ipm_ir_update(opt.info, v.state(), P.n, _lib.dptr(corr_d), _lib.dptr(ezl),
+                                     _lib.dptr(ezu), _lib.dptr(pxd), _lib.dptr(dzl), _lib.dptr(dzu), _s()),
+               "ipm")
+    _, zl, zu, sl, su = S
+    dc = corr[P.pidx]
+    exp_l = dzl_o + np.where(P.fl, (_h(ezl) - zl * dc) / np.where(P.fl, sl, 1.0), 0.0)
+    exp_u = dzu_o + np.where(P.fu, (_h(ezu) + zu * dc) / np.where(P.fu, su, 1.0), 0.0)
+    assert np.allclose(_h(pxd), px + corr, rtol=1e-15, atol=0)
+    assert np.allclose(_h(dzl), exp_l, rtol=1e-13, atol=1e-14)
+    assert np.allclose(_h(dzu), exp_u, rtol=1e-13, atol=1e-14)

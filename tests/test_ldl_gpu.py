@@ -210,3 +210,77 @@ def test_delays_with_wide_fronts_match_oracle():
     x = torch.tensor(b, device="cuda")
     ldl.solve(x)
     assert np.allclose(x.cpu().numpy(), O.solve(b), rtol=1e-9, atol=1e-9)
+
+
+def _tall_front_matrix(nb=1100, tail=48, seed=11):
+    """A dense symmetric indefinite block of nb rows (every tenth diagonal
+    entry zero, so 2x2 pivots occur) followed by a tail of
+    sparsely coupled rows.  In NATURAL order the dense block is cut into
+    64-column supernodes whose first fronts hold more than 1024 rows: the
+    global-memory panel path."""
+    from scipy.sparse import csr_matrix
+
+    rng = np.random.default_rng(seed)
+    B = rng.normal(size=(nb, nb))
+    S = B + B.T
+    S[np.arange(0, nb, 10), np.arange(0, nb, 10)] = 0.0
+    n = nb + tail
+    A = np.zeros((n, n))
+    A[:nb, :nb] = S
+    for t in range(tail):
+        i = nb + t
+        A[i, i] = 3.0 if t % 2 else -2.0
+        for j in rng.choice(nb, 3, replace=False):
+            A[i, j] = A[j, i] = rng.normal()
+        if t:
+            A[i, i - 1] = A[i - 1, i] = 0.5
+    M_ = csr_matrix(A + np.diag(np.full(n, 1e-300)))
+    M_.data[np.abs(M_.data) <= 1e-299] = 0.0
+    M_.sort_indices()
+    return M_
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fronts_taller_than_1024_rows(precision):
+    """The global-memory panel (fronts of more than 1024 rows): inertia
+    equals the eigenvalue inertia and the reference's; in FP64 the per-node
+    pivot and delay counts equal the oracle restatement's; solves reach
+    FP64 residuals (the mixed form after FP64 iterative refinement)."""
+    from amigo_amd.sparse import CSRMat, SparseLDL
+    from oracle.ldl import OracleLDL
+
+    ref = _ref()
+    A = _tall_front_matrix()
+    ldl = SparseLDL(CSRMat.from_scipy(A), order="NATURAL", precision=precision)
+    ldl.factor()
+    assert ldl.stats()[1] > 1024
+    Ad = A.toarray()
+    ev = np.linalg.eigvalsh(Ad)
+    eig = (int(np.sum(ev >= 0)), int(np.sum(ev < 0)))
+    R = ref.RefLDL(A.indptr, A.indices, order="NATURAL")
+    R.factor(A.data)
+    assert ldl.get_inertia() == R.inertia() == eig
+    b = np.random.default_rng(2).normal(size=A.shape[0])
+    x = torch.tensor(b, device="cuda")
+    ldl.solve(x)
+    r = np.linalg.norm(Ad @ x.cpu().numpy() - b) / np.linalg.norm(b)
+    if precision == "fp64":
+        O = OracleLDL(A.shape[0], A.indptr, A.indices, ldl.get_symbolic())
+        O.factor(A.data)
+        npiv, ndel = ldl.pivot_counts()
+        assert O.get_inertia() == eig
+        assert np.array_equal(npiv, O.npiv) and np.array_equal(ndel, O.ndel)
+        np_r, _ = R.pivot_counts()
+        assert np.array_equal(npiv, np_r)
+        assert r < 1e-10
+    else:
+        # FP32 Schur updates: the plain solve is accurate to the FP32 level
+        # only; FP64 refinement on the FP64 matrix recovers FP64 residuals.
+        assert r < 1e-1
+        xh = x.cpu().numpy()
+        for _ in range(15):
+            d = torch.tensor(b - Ad @ xh, device="cuda")
+            ldl.solve(d)
+            xh = xh + d.cpu().numpy()
+        r = np.linalg.norm(Ad @ xh - b) / np.linalg.norm(b)
+        assert r < 1e-12
