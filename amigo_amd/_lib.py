@@ -65,6 +65,8 @@ _SIGS = {
             ctypes.POINTER(ctypes.c_void_p),
         ],
     ),
+    "amigo_chol_create": (ctypes.c_int, [ctypes.c_int, _c_int_p, _c_int_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "amigo_chol_factor": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "amigo_ldl_create_grouped": (
         ctypes.c_int,
         [

@@ -46,6 +46,7 @@ struct amigo_ldl {
   LdlNumeric num;
   bool device_ready = false;
   int scaling = 0;
+  int sqdef = -1;  // quasi-definite split of a SparseCholesky handle (-1: none)
   std::vector<int> rowp, cols;
 };
 
@@ -313,6 +314,30 @@ int amigo_ldl_factor_shifted(amigo_ldl* h, const double* values_dev,
 int amigo_ldl_factor(amigo_ldl* h, const double* values_dev, int precision,
                      void* stream) {
   return amigo_ldl_factor_shifted(h, values_dev, nullptr, precision, stream);
+}
+
+int amigo_chol_create(int n, const int* rowp, const int* cols, int sqdef_index,
+                      amigo_ldl** out) {
+  const int rc = create_handle(n, rowp, cols, AMIGO_SOLVER_CHOLESKY, 0.0, 0.0, 1.0,
+                               AMIGO_ORDER_NATURAL, nullptr, nullptr, out);
+  if (rc) return rc;
+  // SparseCholesky's constructor: a split outside [0, n] means the whole
+  // matrix is the definite block (sparse_cholesky.h:73-76).
+  (*out)->sqdef = (sqdef_index < 0 || sqdef_index > n) ? n : sqdef_index;
+  return AMIGO_OK;
+}
+
+int amigo_chol_factor(amigo_ldl* h, const double* values_dev, void* stream) {
+  if (!h) return fail(AMIGO_ERR_ARG, "amigo_chol_factor: null handle");
+  if (h->solver_type != AMIGO_SOLVER_CHOLESKY || h->sqdef < 0)
+    return fail(AMIGO_ERR_ARG, "amigo_chol_factor: handle not made by amigo_chol_create");
+  int rc = amigo_ldl_factor_shifted(h, values_dev, nullptr, AMIGO_PREC_FP64, stream);
+  if (rc) return rc;
+  std::string err;
+  int bad = 0;
+  rc = h->num.sqd_check(h->sqdef, &bad, static_cast<hipStream_t>(stream), err);
+  if (rc) return fail(rc, "amigo_chol_factor: " + err);
+  return bad;
 }
 
 int amigo_ldl_set_batch(amigo_ldl* h, int num_blocks, const int* node_seg) {

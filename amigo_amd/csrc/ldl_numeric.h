@@ -94,6 +94,10 @@ class LdlNumeric {
     *zero = inertia_[2];
   }
   int pivot_counts(int* np, int* nd, std::string& err) const;
+  // After a factorization without pivoting: 1-based index of the smallest
+  // variable whose pivot has the wrong sign for a quasi-definite matrix
+  // split at sqdef (positive below, negative from sqdef on), or 0.
+  int sqd_check(int sqdef, int* first_bad, hipStream_t stream, std::string& err);
   // Independent diagonal blocks (e.g. many problems in one block-diagonal
   // KKT): node_seg[b]..node_seg[b+1] are block b's nodes.  After each
   // factorization the inertia is also summed per block, and a non-finite
@@ -238,6 +242,7 @@ class LdlNumeric {
   // Pinned host buffers for the device-to-host results and the mask
   // uploads (asynchronous copies whose host side outlives them).
   int* h_flags_pin_ = nullptr;      // [8]
+  int* d_sqd_ = nullptr;            // sign-test result (quasi-definite check)
   int* h_binert_pin_ = nullptr;     // [4 * nbatch]
   char* h_stage_pin_ = nullptr;     // mask / compacted-list upload staging
   size_t h_stage_cap_ = 0;

@@ -3272,6 +3272,50 @@ int LdlNumeric::stage(const void* src, size_t bytes, size_t& off, const void** o
   return 0;
 }
 
+// Sign test of a factorization without pivoting of a symmetric
+// quasi-definite matrix [A B^T; B -C] (SparseCholesky,
+// include/sparse_cholesky.h:9-63): the pivot of a variable below sqdef
+// (the A block) must be positive and that of the others negative.  The
+// smallest 1-based index of a variable that fails goes to out[0] (atomicMin
+// over an INT_MAX start); a non-finite pivot fails too.
+__global__ void sqd_sign_kernel(DevState S, int K, int sqdef, int* __restrict__ out) {
+  const int node = blockIdx.x;
+  if (node >= K || !node_on(S, node)) return;
+  const int p = S.npiv[node], m = S.m[node];
+  const double* F = S.rpool + S.foff[node];
+  const int* vars = S.ipool + S.voff[node];
+  for (int j = threadIdx.x; j < p; j += blockDim.x) {
+    const int q = vars[j];
+    const int v = q >= 0 ? q : -q - 1;
+    const double d = F[j + (size_t)j * m];
+    const bool ok = v < sqdef ? (d > 0.0) : (d < 0.0);
+    if (!ok) atomicMin(out, v + 1);
+  }
+}
+
+int LdlNumeric::sqd_check(int sqdef, int* first_bad, hipStream_t stream, std::string& err) {
+  *first_bad = 0;
+  if (!factored_) {
+    err = "no factorization";
+    return -3;
+  }
+  if (!d_sqd_) {
+    HIPCHK(hipMalloc((void**)&d_sqd_, sizeof(int)));
+    allocs_.push_back(d_sqd_);
+  }
+  if (!h_flags_pin_) HIPCHK(hipHostMalloc((void**)&h_flags_pin_, 8 * sizeof(int)));
+  h_flags_pin_[7] = INT_MAX;
+  HIPCHK(hipMemcpyAsync(d_sqd_, &h_flags_pin_[7], sizeof(int), hipMemcpyHostToDevice, stream));
+  if (K_ > 0)
+    hipLaunchKernelGGL(sqd_sign_kernel, dim3(K_), dim3(64), 0, stream, S_, K_, sqdef, d_sqd_);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(&h_flags_pin_[7], d_sqd_, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  const int h = h_flags_pin_[7];
+  *first_bad = h == INT_MAX ? 0 : h;
+  return 0;
+}
+
 int LdlNumeric::pivot_counts(int* np, int* nd, std::string& err) const {
   HIPCHK(hipMemcpy(np, S_.npiv, K_ * sizeof(int), hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(nd, S_.ndel, K_ * sizeof(int), hipMemcpyDeviceToHost));

@@ -45,7 +45,10 @@ class CSRMat:
     called, also on the device; values live in a float64 device tensor.
     """
 
-    def __init__(self, nrows, ncols, rowp, cols, data=None, device="cuda"):
+    def __init__(self, nrows, ncols, rowp, cols, data=None, device="cuda", sqdef_index=-1):
+        # First variable of the negative definite block of a quasi-definite
+        # matrix (CSRMat's sqdef_index, include/csr_matrix.h:98-113; -1 = none).
+        self.sqdef_index = int(sqdef_index)
         self.nrows = int(nrows)
         self.ncols = int(ncols)
         rowp = np.ascontiguousarray(rowp, dtype=np.int64)
@@ -69,9 +72,10 @@ class CSRMat:
         self._cols_d = None
 
     @classmethod
-    def from_scipy(cls, A, device="cuda"):
+    def from_scipy(cls, A, device="cuda", sqdef_index=-1):
         A = A.tocsr()
-        return cls(A.shape[0], A.shape[1], A.indptr, A.indices, A.data, device=device)
+        return cls(A.shape[0], A.shape[1], A.indptr, A.indices, A.data, device=device,
+                   sqdef_index=sqdef_index)
 
     def diag_indices(self):
         """CSR position of each diagonal entry (-1 when absent)."""
@@ -296,7 +300,18 @@ class SparseLDL:
 
     def solve(self, x):
         """Solve in place.  x: float64 device tensor (n,) or column-major
-        (n, nrhs) (i.e. x.t() contiguous)."""
+        (n, nrhs) (i.e. x.t() contiguous), or a writeable float64 NumPy
+        array (n,) or (n, nrhs) as the reference binding takes
+        (amigo/amigo.cpp:627-689): an F-ordered array is overwritten
+        directly, any other layout through an F-ordered copy that is then
+        copied back.  Errors follow the binding: TypeError for a dtype
+        other than float64, more than two dimensions or a read-only array,
+        RuntimeError for any other object; a first dimension other than n
+        raises ValueError (the reference reads past the array instead)."""
+        if isinstance(x, np.ndarray):
+            return self._solve_numpy(x)
+        if not isinstance(x, torch.Tensor):
+            raise RuntimeError("Expected either a NumPy array or a float64 device tensor")
         if not (x.is_cuda and x.dtype == torch.float64):
             raise TypeError("solve expects a float64 device tensor")
         if x.dim() == 1:
@@ -313,6 +328,28 @@ class SparseLDL:
             )
         )
         return x
+
+    def _solve_numpy(self, arr):
+        if arr.dtype != np.float64:
+            raise TypeError("NumPy array must have dtype=float64")
+        if arr.ndim not in (1, 2):
+            raise TypeError("NumPy array must be 1D or 2D")
+        if not arr.flags.writeable:
+            raise TypeError("NumPy array must be writeable")
+        if arr.shape[0] != self.n:
+            raise ValueError(f"array has {arr.shape[0]} rows, the matrix {self.n}")
+        nrhs = 1 if arr.ndim == 1 else arr.shape[1]
+        if nrhs == 0:
+            return arr
+        host = np.asfortranarray(arr)
+        dev = torch.from_numpy(host.reshape(-1, order="F")).to(self.mat.device)
+        _lib.check(
+            _lib.lib().amigo_ldl_solve(self._h, nrhs, _lib.vp(dev.data_ptr()), self.n,
+                                       _lib.vp(current_stream_ptr()))
+        )
+        out = dev.cpu().numpy().reshape(arr.shape, order="F")
+        np.copyto(arr, out)
+        return arr
 
     def stats(self):
         """(delayed pivots, largest front) of the last factorization."""
@@ -349,9 +386,28 @@ class SparseLDL:
 
 
 class SparseCholesky(SparseLDL):
-    """Drop-in for SparseCholesky<T> (include/sparse_cholesky.h:64-376):
-    a factorization without pivoting of a symmetric (quasi-)definite matrix.
-    The reference orders naturally; so does this one."""
+    """Drop-in for SparseCholesky<T> (include/sparse_cholesky.h:64-376): a
+    factorization without pivoting, in natural order, of a symmetric
+    quasi-definite matrix [A B^T; B -C] whose first ``mat.sqdef_index``
+    variables form the positive definite block A (-1 or n: the whole matrix
+    is positive definite).  ``factor()`` returns 0, or a positive value
+    when a pivot has the wrong sign for its block, like the reference's
+    failing-column return."""
 
     def __init__(self, mat):
-        super().__init__(mat, solver_type=CHOLESKY, order="NATURAL")
+        h = ctypes.c_void_p()
+        _lib.check(
+            _lib.lib().amigo_chol_create(
+                mat.nrows, _lib.ip(mat.rowp), _lib.ip(mat.cols), int(getattr(mat, "sqdef_index", -1)),
+                ctypes.byref(h),
+            )
+        )
+        super().__init__(mat, solver_type=CHOLESKY, order="NATURAL", _handle=h)
+
+    def factor(self):
+        rc = _lib.lib().amigo_chol_factor(self._h, _lib.vp(self.mat.data.data_ptr()),
+                                          _lib.vp(current_stream_ptr()))
+        if rc < 0:
+            _lib.check(rc)
+        self._factored = True
+        return int(rc)

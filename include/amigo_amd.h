@@ -138,6 +138,22 @@ int amigo_ldl_factor_shifted(amigo_ldl* h, const double* values_dev,
 int amigo_ldl_get_inertia(const amigo_ldl* h, int* npos, int* nneg,
                           int* nzero);
 
+/* SparseCholesky<T>::SparseCholesky(mat) (include/sparse_cholesky.h:67-136)
+ * for a symmetric quasi-definite matrix [A B^T; B -C] whose first
+ * sqdef_index variables form the definite block A (CSRMat's sqdef_index,
+ * include/csr_matrix.h:689; a value outside [0, n] means the whole matrix
+ * is definite, as sparse_cholesky.h:73-76).  Natural order, no pivoting. */
+int amigo_chol_create(int n, const int* rowp, const int* cols, int sqdef_index,
+                      amigo_ldl** out);
+/* SparseCholesky<T>::factor() (include/sparse_cholesky.h:180-306): returns
+ * 0 when every pivot has the sign of its block (positive in A, negative in
+ * -C), otherwise the 1-based index of the smallest variable whose pivot
+ * fails (the reference returns the failing column of its left-looking
+ * order; both are positive exactly when the matrix is not quasi-definite
+ * in this split), or a negative AMIGO_ERR_* code.  Solve with
+ * amigo_ldl_solve (sparse_cholesky.h:311-376). */
+int amigo_chol_factor(amigo_ldl* h, const double* values_dev, void* stream);
+
 /* Block-diagonal batches (many independent KKT matrices factored as one):
  * node_seg (host, num_blocks+1 entries, from 0 to num_nodes) gives each
  * block's contiguous node range.  After every factorization the inertia is
