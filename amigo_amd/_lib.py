@@ -205,6 +205,7 @@ _SIGS.update(
         "amigo_ipm_max_step": (_I, [_PP, _PS, _D, _V, _V, _V, _V, _c_dbl_p, _V]),
         "amigo_ipm_apply_step": (_I, [_PP, _PS, _D, _D, _V, _V, _V, _I, _PS, _V]),
         "amigo_ipm_reduce": (_I, [_I, _PP, _PS, _D, _V, _V, _V, _V, _c_dbl_p, _V]),
+        "amigo_ipm_reduce_noalloc": (_I, [_I, _PP, _PS, _D, _V, _V, _V, _c_dbl_p, _V]),
         "amigo_ipm_reset_bound_multipliers": (_I, [_PP, _PS, _D, _D, _V]),
         "amigo_ipm_dual_residual_vector": (_I, [_PP, _PS, _V, _V, _I, _V]),
         "amigo_ipm_affine_start_point": (_I, [_PP, _PS, _D, _V, _V, _V, _V, _V]),

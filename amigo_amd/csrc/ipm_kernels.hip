@@ -11,6 +11,7 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "../../include/amigo_ipm.h"
@@ -1069,6 +1070,43 @@ __global__ __launch_bounds__(NT) void segment_absmax_kernel(const double* __rest
   }
 }
 
+// ---- scalar reductions without allocation -----------------------------------
+// The workspace of amigo_ipm_reduce_noalloc: the per-block partial rows of one
+// reduction (at most MAXB blocks x 4 values) followed by the 4 combined
+// results, as in a caller's ws.  One copy per GPU, held until the process
+// ends; every call on that GPU reuses it.
+constexpr int kScratchDoubles = MAXB * 4 + 16;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.  In host code the array's name is only a
+// stand-in for the symbol; kernels get this address.
+int scratch_on_current_device(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_err("hipGetDevice", e);
+  if (dev < 0 || dev >= kMaxDevices) {
+    g_ipm_err = "amigo_ipm_reduce_noalloc: device index " + std::to_string(dev) +
+                " beyond the " + std::to_string(kMaxDevices) + " supported";
+    return -1;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return set_err("hipGetSymbolAddress(scratch)", e);
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
 extern "C" {
 
 const char* amigo_ipm_last_error(void) { return g_ipm_err.c_str(); }
@@ -1148,6 +1186,40 @@ int amigo_ipm_reduce(int kind, const amigo_ipm_problem* P, const amigo_ipm_state
   hipStream_t s = (hipStream_t)stream;
   const int total = P->n_primal + P->n_constraints;
   const int nb = nblocks(total);
+  hipLaunchKernelGGL(reduce_kernel, dim3(nb), dim3(NT), 0, s, BT{}, kind, *P, *S, mu, nullptr,
+                     a, b, c, ws);
+  LAUNCH_CHECK("reduce_kernel");
+  int ops[4];
+  reduce_ops(kind, ops);
+  return finish_and_copy(4, ops, ws, nb, out4, s);
+}
+
+int amigo_ipm_reduce_noalloc(int kind, const amigo_ipm_problem* P, const amigo_ipm_state* S,
+                             double mu, const double* a, const double* b, const double* c,
+                             double* out4, void* stream) {
+  if (!P || !S || !out4) {
+    g_ipm_err = "amigo_ipm_reduce_noalloc: null problem, state or output";
+    return -1;
+  }
+  if (kind < RK_COMPL || kind > RK_ABS_SUMS) {
+    g_ipm_err = "amigo_ipm_reduce_noalloc: unknown reduction kind " + std::to_string(kind);
+    return -1;
+  }
+  const long long total = (long long)P->n_primal + P->n_constraints;
+  if (P->n_primal < 0 || P->n_constraints < 0 || total > INT_MAX) {
+    g_ipm_err = "amigo_ipm_reduce_noalloc: n_primal + n_constraints must lie in [0, 2^31 - 1]";
+    return -1;
+  }
+  const int nb = nblocks((int)total);
+  if ((long long)nb * 4 + 4 > kScratchDoubles) {
+    g_ipm_err = "amigo_ipm_reduce_noalloc: " + std::to_string(nb) +
+                " partial blocks exceed the static workspace";
+    return -1;
+  }
+  double* ws = nullptr;
+  const int rc = scratch_on_current_device(&ws);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(reduce_kernel, dim3(nb), dim3(NT), 0, s, BT{}, kind, *P, *S, mu, nullptr,
                      a, b, c, ws);
   LAUNCH_CHECK("reduce_kernel");

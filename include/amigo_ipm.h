@@ -87,6 +87,22 @@ int amigo_ipm_reduce(int kind, const amigo_ipm_problem* P,
                      const double* b, const double* c, double* ws, double* out4,
                      void* stream);
 
+/* amigo_ipm_reduce without the ws argument and without any allocation: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The partial sums go to a static device array of
+ * 4 x 1024 + 16 doubles (32.9 KB) that each GPU holds until the process
+ * ends.  Any n_primal + n_constraints up to 2^31 - 1 fits it (the grid is
+ * capped at 1024 blocks); larger or negative sizes, an unknown kind or a
+ * null pointer return -1 with a message.  All calls on one GPU share its
+ * array: calls that overlap on one GPU (other streams, other threads) race
+ * and must be serialized by the caller.  Results equal amigo_ipm_reduce's
+ * bit for bit. */
+int amigo_ipm_reduce_noalloc(int kind, const amigo_ipm_problem* P,
+                             const amigo_ipm_state* S, double mu, const double* a,
+                             const double* b, const double* c, double* out4,
+                             void* stream);
+
 /* ipm::reset_bound_multipliers (backend.h:459-476), in place. */
 int amigo_ipm_reset_bound_multipliers(const amigo_ipm_problem* P,
                                       const amigo_ipm_state* S, double mu,
