@@ -23,7 +23,13 @@ SOURCES = [
     "ipm_kernels.hip",
     "assembly.hip",
     "capi.cpp",
+    "driver_kernels.hip",
+    "ipm_driver.cpp",
 ]
+
+# Host decisions of the driver compare values computed with the same
+# operations as the reference's Python driver: no contraction of a*b+c.
+EXTRA_FLAGS = {"ipm_driver.cpp": ["-ffp-contract=off"]}
 
 ARCH = "gfx950"
 
@@ -58,9 +64,10 @@ def common_flags():
 def compile_one(src):
     path = os.path.join(CSRC, src)
     obj = os.path.join(BUILD, src + ".o")
-    cmd = [hipcc()] + common_flags() + ["-c", path, "-o", obj]
+    extra = EXTRA_FLAGS.get(src, [])
+    cmd = [hipcc()] + common_flags() + extra + ["-c", path, "-o", obj]
     if src.endswith(".cpp"):
-        cmd = [hipcc(), "-x", "hip"] + common_flags() + ["-c", path, "-o", obj]
+        cmd = [hipcc(), "-x", "hip"] + common_flags() + extra + ["-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {src}\n{r.stdout}\n{r.stderr}")

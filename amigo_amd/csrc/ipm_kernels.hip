@@ -1018,18 +1018,23 @@ __global__ void scatter_scale_kernel(BT B, int nc, const int* __restrict__ cidx,
 // sl, su only, 3 -> all five vectors (OptVector::copy of the problems that
 // take part).  Null destination members are skipped.
 __global__ void copy_state_kernel(BT B, amigo_ipm_state D, amigo_ipm_state S, int what) {
+  // A member is copied only when both states have it: the flags are taken
+  // before the views shift the pointers by the problem's offsets.
+  const bool cx = (what & 1) && D.xlam && S.xlam;
+  const bool czl = (what & 2) && D.zl && S.zl, czu = (what & 2) && D.zu && S.zu;
+  const bool csl = (what & 2) && D.sl && S.sl, csu = (what & 2) && D.su && S.su;
   amigo_ipm_problem P{0, 0, nullptr, nullptr, nullptr, nullptr, nullptr};
   const View V = view(B, P, S);
   if (V.skip) return;
   shift_state(D, V);
   const int np = B.pseg[V.k + 1] - V.p0;
   for (int t = V.bx * NT + threadIdx.x; t < V.n; t += V.gx * NT) {
-    if (what & 1) D.xlam[t] = S.xlam[t];
-    if ((what & 2) && t < np) {
-      D.zl[t] = S.zl[t];
-      D.zu[t] = S.zu[t];
-      D.sl[t] = S.sl[t];
-      D.su[t] = S.su[t];
+    if (cx) D.xlam[t] = S.xlam[t];
+    if (t < np) {
+      if (czl) D.zl[t] = S.zl[t];
+      if (czu) D.zu[t] = S.zu[t];
+      if (csl) D.sl[t] = S.sl[t];
+      if (csu) D.su[t] = S.su[t];
     }
   }
 }

@@ -288,7 +288,8 @@ int amigo_ipm_batch_ir_update(const amigo_ipm_batch* B, const amigo_ipm_problem*
 int amigo_segment_absmax(const double* x, const int* seg, int K, double* out_dev,
                          double* out_host, void* stream);
 /* OptVector::copy (include/interior_point_optimizer.h:21-90) of the active
- * problems' segments: what = 1 x only, 2 zl/zu/sl/su only, 3 all. */
+ * problems' segments: what = 1 x only, 2 zl/zu/sl/su only, 3 all.  A member
+ * that is NULL in either state is skipped. */
 int amigo_ipm_batch_copy_state(const amigo_ipm_batch* B, const amigo_ipm_state* dst,
                                const amigo_ipm_state* src, int what, void* stream);
 

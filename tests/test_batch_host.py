@@ -1,89 +1,13 @@
-"""Host logic of the batched driver (CPU): the vectorized inertia corrector
-and filters against the single-problem classes they restate, the union
-problem's pattern / gather lists, and the union symbolic plan against each
-problem's own analysis."""
+"""Host logic of the batched form (CPU): the union problem's pattern and
+gather lists, and the union symbolic plan against each problem's own
+analysis."""
 
 import numpy as np
 import pytest
 
-from amigo_amd.batch import BatchFilter, BatchIC, BatchProblem, problem_ldl
-from amigo_amd.optimizer import Filter, InertiaCorrector
+from amigo_amd.batch import BatchProblem, problem_ldl
 from amigo_amd.problems import cart_pole, double_integrator
 from amigo_amd.sparse import CSRMat, SparseLDL
-
-
-def _ic_state(c):
-    return (c._delta_x_last, c._delta_c_last, c._delta_x_curr, c._delta_c_curr, c._hess_degen,
-            c._jac_degen, c._degen_iters, c._test_status, c._pivtol)
-
-
-def _bic_state(b, k):
-    return (b.dx_last[k], b.dc_last[k], b.dx_curr[k], b.dc_curr[k], b.hess_degen[k], b.jac_degen[k],
-            b.degen_iters[k], b.test[k], b.pivtol[k])
-
-
-@pytest.mark.parametrize("seed", range(6))
-def test_batch_ic_matches_single_problem_ic(seed):
-    rng = np.random.default_rng(seed)
-    K = 7
-    mus = rng.uniform(1e-9, 1.0, K)
-    singles = [InertiaCorrector(np.zeros(1, bool), np.zeros(1, bool), mus[k], {}) for k in range(K)]
-    b = BatchIC(K, mus)
-    for step in range(300):
-        I = np.sort(rng.choice(K, size=rng.integers(1, K + 1), replace=False))
-        op = rng.integers(0, 5)
-        if op == 0:
-            b.consider_new_system(I)
-            for k in I:
-                singles[k]._consider_new_system()
-        elif op == 1:
-            r = b.perturb_wrong_inertia(I)
-            for t, k in enumerate(I):
-                assert r[t] == singles[k]._perturb_for_wrong_inertia()
-        elif op == 2:
-            r = b.perturb_singularity(I)
-            for t, k in enumerate(I):
-                assert r[t] == singles[k]._perturb_for_singularity()
-        elif op == 3:
-            r = b.increase_quality(I)
-            for t, k in enumerate(I):
-                assert r[t] == singles[k]._increase_quality(None)
-        else:
-            new = rng.uniform(1e-9, 1.0, K)
-            b.mu[I] = new[I]
-            for k in I:
-                singles[k].update_barrier(new[k])
-        for k in range(K):
-            assert _bic_state(b, k) == _ic_state(singles[k]), (step, op, k)
-
-
-@pytest.mark.parametrize("seed", range(4))
-def test_batch_filter_matches_filter(seed):
-    rng = np.random.default_rng(seed)
-    K = 5
-    singles = [Filter() for _ in range(K)]
-    b = BatchFilter(K, cap=2)
-    for step in range(200):
-        I = np.sort(rng.choice(K, size=rng.integers(1, K + 1), replace=False))
-        phi = rng.normal(size=len(I))
-        th = rng.uniform(0, 2, size=len(I))
-        op = rng.integers(0, 4)
-        if op < 2:
-            b.add(I, phi, th)
-            for t, k in enumerate(I):
-                singles[k].add(phi[t], th[t])
-        elif op == 2:
-            got = b.acceptable(I, phi, th)
-            for t, k in enumerate(I):
-                assert got[t] == singles[k].is_acceptable(phi[t], th[t])
-        else:
-            b.clear(I[:1])
-            singles[I[0]].clear()
-        sz = b.size()
-        for k in range(K):
-            assert sz[k] == len(singles[k])
-            ent = sorted(zip(b.phi[k][b.valid[k]], b.theta[k][b.valid[k]]))
-            assert ent == sorted(singles[k].entries)
 
 
 def _problems():
