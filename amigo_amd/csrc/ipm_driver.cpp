@@ -331,6 +331,15 @@ struct Prob {
   double res_norm = 0.0, ax = 0.0, az = 0.0;
   long long xi = -1, zi = -1;
   double ir_ratio = 0.0;
+  double ls_ratio = 0.0;  // step fraction the line search settled on
+  int iter = 0;           // main iterations completed
+  // feasibility restoration by Newton steps at raised barrier parameters
+  // (feasibility_restoration.py:102-172), one restoration iteration per round
+  bool restoring = false, restore_ok = false;
+  std::vector<double> rlevels;
+  int rlevel = 0, riter = 0, rper_level = 0;
+  double rsaved_mu = 0.0, rtheta = 0.0, rtheta_start = 0.0, rtarget = 0.0;
+  double rax = 0.0, raz = 0.0;
   std::vector<amigo_ipm_iteration> records;
   std::vector<int> inertia, solver_inertia;
   int fp64_fallbacks = 0, ir_steps = 0, restorations = 0;
@@ -677,13 +686,15 @@ struct amigo_ipm_driver {
   std::vector<char> factorize(const Set& A);
   void solve_with_mu(const Set& B);
   std::vector<double> refine(const Set& B);
-  void barrier_update(int it, const Set& A);
+  void barrier_update(const Set& A);
   void line_search_with_watchdog(const Set& B, std::vector<Search>& out);
   void line_search(const Set& S, std::vector<Search>& ls, const std::vector<double>& phi_cur,
                    const Set& wd_ref);
   void accept_checks(Prob& p, Search& L, double tb, double tt, double a, bool& acc, bool& frej) const;
   void update_filter(Prob& p, const Search& L, double tb, double a) const;
-  Set restoration(const Set& R);
+  Set restoration_begin(const Set& R);
+  Set restoration_search(const Set& RB);
+  void finish_step(const Set& done, const Set& accepted);
   double tau_of(double mu) const {
     return o.adaptive_tau ? std::max(o.tau_min, 1.0 - mu) : o.fraction_to_boundary;
   }
@@ -1041,15 +1052,14 @@ void amigo_ipm_driver::solve_with_mu(const Set& B) {
   direction(B, px.p);
 }
 
-// Barrier update of the problems of A at iteration it: the LOQO rule of
+// Barrier update of the problems of A past their first iteration: the LOQO rule of
 // barrier_strategy/heuristic.py:9-50, or the monotone rule with its
 // progress test.
-void amigo_ipm_driver::barrier_update(int it, const Set& A) {
-  if (it == 0) return;
+void amigo_ipm_driver::barrier_update(const Set& A) {
   const double mu_floor =
       std::min(o.convergence_tolerance, o.compl_inf_tol) / (o.barrier_tol_factor + 1.0);
   Set M(K, 0);
-  for (int k = 0; k < K; k++) M[k] = A[k] && P[k].mu > mu_floor;
+  for (int k = 0; k < K; k++) M[k] = A[k] && P[k].iter > 0 && P[k].mu > mu_floor;
   if (!any(M)) return;
   if (o.barrier_heuristic) {
     Red c = reduce(0, M, vars, zeros(), nullptr);
@@ -1440,18 +1450,21 @@ void amigo_ipm_driver::line_search_with_watchdog(const Set& B, std::vector<Searc
 }
 
 // ---- feasibility restoration (feasibility_restoration.py:16-172) -------------
-// Near-feasible problems backtrack on the residual norm against the filter;
-// the others take Newton steps at raised barrier parameters until theta
-// falls by 10 %.  Returns the problems whose restoration succeeded.
-Set amigo_ipm_driver::restoration(const Set& R) {
+// Near-feasible problems (theta below max(tol, 1e-4 max(1, theta_0)))
+// backtrack on the residual norm against the filter at once.  The others
+// enter the restoration phase: Newton steps at raised barrier parameters
+// until theta falls by 10 %, one step per round, inside the same launches as
+// the problems in their main iterations.  Returns the problems whose
+// restoration is already complete (all of them succeed).
+Set amigo_ipm_driver::restoration_begin(const Set& R) {
   Set success(K, 0);
   Red th = reduce(4, R, vars, zeros(), grad.p);
   fetch();
-  std::vector<double> theta_k(K, 0.0), theta_start(K, 0.0);
+  std::vector<double> theta_k(K, 0.0);
   Set RA(K, 0), RB(K, 0);
   for (int k = 0; k < K; k++) {
     if (!R[k]) continue;
-    theta_k[k] = theta_start[k] = at(th, k);
+    theta_k[k] = at(th, k);
     const double t0 = std::isnan(P[k].theta0) ? theta_k[k] : P[k].theta0;
     const double theta_min_restore = 1e-4 * std::max(1.0, t0);
     if (theta_k[k] < std::max(o.convergence_tolerance, theta_min_restore))
@@ -1532,208 +1545,285 @@ Set amigo_ipm_driver::restoration(const Set& R) {
     Red ob = objective(RB, vars.x);
     Red lg = reduce(2, RB, vars, scalars(per(&Prob::mu)), nullptr);
     fetch();
-    std::vector<std::vector<double>> levels(K);
-    std::vector<int> level(K, 0), iter(K, 0), per_level(K, 0);
-    std::vector<double> saved(K), target(K);
     for (int k = 0; k < K; k++) {
       if (!RB[k]) continue;
-      P[k].filter.add(at(ob, k) + at(lg, k), theta_k[k]);
-      target[k] = 0.9 * theta_k[k];
-      saved[k] = P[k].mu;
+      Prob& p = P[k];
+      p.filter.add(at(ob, k) + at(lg, k), theta_k[k]);
+      p.rtheta = p.rtheta_start = theta_k[k];
+      p.rtarget = 0.9 * theta_k[k];
+      p.rsaved_mu = p.mu;
       std::vector<double> lv;
-      for (double m : {std::max(saved[k], 1e-2), 0.1, 1.0})
-        if (m >= saved[k]) lv.push_back(m);
+      for (double m : {std::max(p.mu, 1e-2), 0.1, 1.0})
+        if (m >= p.mu) lv.push_back(m);
       std::sort(lv.begin(), lv.end());
       lv.erase(std::unique(lv.begin(), lv.end()), lv.end());
-      levels[k] = lv;
-      per_level[k] = std::max(3, o.filter_restoration_max_iter / int(lv.size()));
+      p.rlevels = lv;
+      p.rlevel = 0;
+      p.riter = 0;
+      p.rper_level = std::max(3, o.filter_restoration_max_iter / int(lv.size()));
+      p.restoring = true;
     }
-    Set act = RB;
-    while (any(act)) {
-      for (int k = 0; k < K; k++)
-        if (act[k]) {
-          P[k].mu = levels[k][level[k]];
-          P[k].reg.set_mu(P[k].mu);
-        }
-      gradient(act, vars.x, grad.p);
-      sigma(act, vars, diag.p);
-      copy_x(act, dbase.p, diag.p);
-      std::vector<char> ok = factorize(act);
-      Set good(K, 0);
-      for (int k = 0; k < K; k++) good[k] = act[k] && ok[k];
-      if (any(good)) solve_with_mu(good);
-      std::vector<double> tau(K);
-      for (int k = 0; k < K; k++) tau[k] = tau_of(P[k].mu);
-      Red ms = max_step(act, tau);
-      fetch();
-      std::vector<double> ax(K), az(K), alpha(K, 1.0);
-      for (int k = 0; k < K; k++)
-        if (act[k]) {
-          ax[k] = at(ms, k, 0);
-          az[k] = at(ms, k, 2);
-        }
-      Set srch = act, taken(K, 0);
-      for (int t = 0; t < max_ls && any(srch); t++) {
-        std::vector<double> a(K, 0.0), b(K, 0.0);
-        for (int k = 0; k < K; k++) {
-          a[k] = alpha[k] * ax[k];
-          b[k] = alpha[k] * az[k];
-        }
-        trial(srch, a, b);
-        gradient(srch, temp.x, grad.p);
-        Red tt = reduce(4, srch, temp, zeros(), grad.p);
-        fetch();
-        Set take(K, 0);
-        for (int k = 0; k < K; k++) {
-          if (!srch[k]) continue;
-          const double v = at(tt, k);
-          if (v < theta_k[k]) {
-            take[k] = 1;
-            taken[k] = 1;
-            theta_k[k] = v;
-            srch[k] = 0;
-          } else {
-            alpha[k] *= backtrack;
-          }
-        }
-        if (any(take)) copy_state(take, vars, temp, 3);
-      }
-      for (int k = 0; k < K; k++) {
-        if (!act[k]) continue;
-        bool next_level = false;
-        if (!taken[k] || theta_k[k] < target[k]) {
-          if (theta_k[k] < target[k]) {
-            act[k] = 0;
-            continue;
-          }
-          next_level = true;
-        } else if (++iter[k] >= per_level[k]) {
-          next_level = true;
-        }
-        if (next_level) {
-          iter[k] = 0;
-          if (++level[k] >= int(levels[k].size())) act[k] = 0;
-        }
-      }
-    }
-    for (int k = 0; k < K; k++)
-      if (RB[k]) {
-        P[k].mu = saved[k];
-        P[k].reg.set_mu(saved[k]);
-      }
-    gradient(RB, vars.x, grad.p);
-    sigma(RB, vars, diag.p);
-    for (int k = 0; k < K; k++)
-      if (RB[k]) success[k] = theta_k[k] < theta_start[k] * 0.99;
   }
   return success;
 }
 
+// The backtracking of one restoration iteration (its Newton step and step
+// bounds are computed with the round's other problems): the first trial that
+// lowers theta is taken.  Returns the problems whose restoration ended
+// (restore_ok tells whether theta fell by 1 %).
+Set amigo_ipm_driver::restoration_search(const Set& RB) {
+  const int max_ls = o.max_line_search_iterations;
+  const double backtrack = o.backtracking_factor;
+  std::vector<double> alpha(K, 1.0);
+  Set srch = RB, taken(K, 0);
+  for (int t = 0; t < max_ls && any(srch); t++) {
+    std::vector<double> a(K, 0.0), b(K, 0.0);
+    for (int k = 0; k < K; k++) {
+      a[k] = alpha[k] * P[k].rax;
+      b[k] = alpha[k] * P[k].raz;
+    }
+    trial(srch, a, b);
+    gradient(srch, temp.x, grad.p);
+    Red tt = reduce(4, srch, temp, zeros(), grad.p);
+    fetch();
+    Set take(K, 0);
+    for (int k = 0; k < K; k++) {
+      if (!srch[k]) continue;
+      const double v = at(tt, k);
+      if (v < P[k].rtheta) {
+        take[k] = 1;
+        taken[k] = 1;
+        P[k].rtheta = v;
+        srch[k] = 0;
+      } else {
+        alpha[k] *= backtrack;
+      }
+    }
+    if (any(take)) copy_state(take, vars, temp, 3);
+  }
+  Set ended(K, 0);
+  for (int k = 0; k < K; k++) {
+    if (!RB[k]) continue;
+    Prob& p = P[k];
+    bool next_level = false, stop = false;
+    if (!taken[k] || p.rtheta < p.rtarget) {
+      if (p.rtheta < p.rtarget)
+        stop = true;
+      else
+        next_level = true;
+    } else if (++p.riter >= p.rper_level) {
+      next_level = true;
+    }
+    if (next_level) {
+      p.riter = 0;
+      if (++p.rlevel >= int(p.rlevels.size())) stop = true;
+    }
+    if (stop) ended[k] = 1;
+  }
+  if (any(ended)) {
+    for (int k = 0; k < K; k++)
+      if (ended[k]) {
+        Prob& p = P[k];
+        p.mu = p.rsaved_mu;
+        p.reg.set_mu(p.mu);
+        p.restoring = false;
+        p.restore_ok = p.rtheta < p.rtheta_start * 0.99;
+      }
+    gradient(ended, vars.x, grad.p);
+    sigma(ended, vars, diag.p);
+  }
+  return ended;
+}
+
+// The end of a main iteration (ipm_driver.py:430-473) for the problems of
+// done whose Newton step was found: an accepted step floors the slacks,
+// resets the bound multipliers and refreshes the gradient; a rejected one
+// restores mu.  The records of the next iteration read the step.
+void amigo_ipm_driver::finish_step(const Set& done, const Set& accepted) {
+  Set G = done & accepted;
+  if (any(G)) {
+    std::vector<double> smin(K, 0.0);
+    for (int k = 0; k < K; k++) {
+      double sm = kEps * std::min(1.0, P[k].mu);
+      if (sm == 0.0) sm = std::numeric_limits<double>::min();
+      smin[k] = sm;
+    }
+    const amigo_ipm_batch BG = batch(mask(G));
+    const amigo_ipm_problem I = info();
+    const amigo_ipm_state S = vars.c();
+    Red fl = red_slot(1);
+    ipm_ok(amigo_ipm_batch_floor_slacks(&BG, &I, &S, d.lbx0, d.ubx0, scalars(per(&Prob::mu)),
+                                        scalars(smin), 1.81898940354586e-12, ws.p, ex.dev(fl.off),
+                                        nullptr, s),
+           "slack floor");
+    ipm_ok(amigo_ipm_batch_reset_bound_multipliers(&BG, &I, &S, scalars(per(&Prob::mu)), 1e10, s),
+           "multiplier reset");
+    gradient(G, vars.x, grad.p);
+  }
+  for (int k = 0; k < K; k++) {
+    if (!done[k]) continue;
+    Prob& p = P[k];
+    if (!accepted[k]) {
+      p.rec_ax = p.rec_az = 0.0;
+      p.rec_xi = p.rec_zi = -1;
+      p.mu = p.mu_before;
+      increase_barrier(p);
+    } else {
+      p.rec_ax = p.ls_ratio * p.ax;
+      p.rec_az = p.az;
+      p.rec_xi = p.xi;
+      p.rec_zi = p.zi;
+      p.rejections = 0;
+      if (!p.wd_on) {
+        if (p.ls_ratio == 1.0 || p.line_iters == 1)
+          p.wd_shortened = 0;
+        else
+          p.wd_shortened++;
+      }
+    }
+    p.iter++;
+  }
+}
+
 // ---- the main loop (ipm_driver.py:106-483) --------------------------------------
+// Each round, every running problem takes one step of its own: a main
+// iteration (records and convergence test, barrier update, Newton
+// direction, line search), or one restoration iteration.  The Newton
+// systems of both kinds are factored and solved together.
 void amigo_ipm_driver::run() {
   t0 = std::chrono::steady_clock::now();
   auto tp = t0;
   reset();
   initialize();
   toc(PH_INIT, tp);
-  const double* z0 = nullptr;
-  for (int it = 0; it < o.max_iterations; it++) {
-    Set A(K, 0);
-    for (int k = 0; k < K; k++) A[k] = P[k].running;
-    if (!any(A)) break;
-    z0 = zeros();
-    for (int k = 0; k < K; k++)
-      if (A[k]) P[k].reg.set_mu(P[k].mu);
-    Red rr = residual(A, vars, grad.p, res.p);
-    Set need(K, 0);
-    for (int k = 0; k < K; k++) need[k] = A[k] && std::isnan(P[k].theta0);
-    Red th0{};
-    if (any(need)) th0 = reduce(4, need, vars, z0, grad.p);
-    Red kkt = reduce(1, A, vars, z0, grad.p);
-    Red sums = reduce(9, A, vars, z0, nullptr);
-    Red xm = red_slot(1);
-    ipm_ok(amigo_segment_absmax(vars.x, d.xseg, K, ex.dev(xm.off), nullptr, s), "absmax");
-    Red ob{}, lg{};
-    if (o.record_objective) {
-      ob = objective(A, vars.x);
-      lg = reduce(2, A, vars, scalars(per(&Prob::mu)), nullptr);
-    }
-    fetch();
-    const double now = elapsed();
+  auto stop = [&](int k, double now) {
+    P[k].running = false;
+    P[k].t_end = now;
+    if (on_done) on_done(ctx, k);
+  };
+  while (true) {
+    Set M(K, 0), RB(K, 0), limit(K, 0);
     for (int k = 0; k < K; k++) {
-      if (!A[k]) continue;
-      Prob& p = P[k];
-      p.res_norm = std::sqrt(at(rr, k, 0));
-      if (need[k]) p.theta0 = at(th0, k);
-      const double du = at(kkt, k, 0), pr = at(kkt, k, 1), co = at(kkt, k, 2);
-      double s_d, s_c;
-      optimality_scaling(at(sums, k, 0), at(sums, k, 1), at(sums, k, 2), at(sums, k, 3), s_d, s_c);
-      const double err = std::max(std::max(du / s_d, pr), co / s_c);
-      int status = AMIGO_IPM_RUNNING;
-      if (err <= o.convergence_tolerance && du <= o.dual_inf_tol && pr <= o.constr_viol_tol &&
-          co <= o.compl_inf_tol) {
-        status = AMIGO_IPM_CONVERGED;
-      } else if (at(xm, k) > o.diverging_iterates_tol) {
-        status = AMIGO_IPM_DIVERGED;
-      } else {
-        const bool acceptable = err <= o.acceptable_tol && du <= o.acceptable_dual_inf_tol &&
-                                pr <= o.acceptable_constr_viol_tol && co <= o.acceptable_compl_inf_tol;
-        if (o.acceptable_iter > 0 && acceptable) {
-          if (++p.acc_count >= o.acceptable_iter) status = AMIGO_IPM_ACCEPTABLE;
-        } else {
-          p.acc_count = 0;
-        }
-        if (status == AMIGO_IPM_RUNNING) {
-          const double rel = std::fabs(p.res_norm - p.prev_res) / std::max(p.res_norm, 1e-30);
-          p.floor_count = (rel < 1e-14 && it > 0) ? p.floor_count + 1 : 0;
-          if (p.floor_count >= 3 && acceptable) status = AMIGO_IPM_PRECISION_FLOOR;
-        }
-      }
-      amigo_ipm_iteration rec{};
-      rec.iteration = it;
-      rec.time = now;
-      rec.residual = p.res_norm;
-      rec.barrier_param = p.mu;
-      rec.line_iters = p.line_iters;
-      rec.alpha_x = p.rec_ax;
-      rec.x_index = int(p.rec_xi);
-      rec.alpha_z = p.rec_az;
-      rec.z_index = int(p.rec_zi);
-      rec.inertia_delta = p.last_dw;
-      rec.inf_du = du;
-      rec.inf_pr = pr;
-      rec.compl_inf = co;
-      rec.nlp_error = err;
-      rec.filter_size = p.filter.size();
-      rec.objective = o.record_objective ? at(ob, k) + at(lg, k) : kNaN;
-      p.records.push_back(rec);
-      if (status != AMIGO_IPM_RUNNING) {
-        p.status = status;
-        p.running = false;
-        p.t_end = now;
-        A[k] = 0;
-        if (on_done) on_done(ctx, k);
-      }
+      if (!P[k].running) continue;
+      if (P[k].restoring)
+        RB[k] = 1;
+      else if (P[k].iter >= o.max_iterations)
+        limit[k] = 1;
+      else
+        M[k] = 1;
     }
-    if (!any(A)) break;
+    if (any(limit)) {
+      // the last step's launches finish before the iterate is handed over
+      hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
+      const double now = elapsed();
+      for (int k = 0; k < K; k++)
+        if (limit[k]) stop(k, now);
+    }
+    if (!any(M) && !any(RB)) break;
+    // restoration iterations: the barrier parameter of their level
     for (int k = 0; k < K; k++)
-      if (A[k]) P[k].prev_res = P[k].res_norm;
-    sigma(A, vars, diag.p);
-    copy_x(A, dbase.p, diag.p);
+      if (RB[k]) {
+        P[k].mu = P[k].rlevels[P[k].rlevel];
+        P[k].reg.set_mu(P[k].mu);
+      }
+    if (any(RB)) gradient(RB, vars.x, grad.p);
+    // main iterations: records and the convergence test
+    if (any(M)) {
+      for (int k = 0; k < K; k++)
+        if (M[k]) P[k].reg.set_mu(P[k].mu);
+      const double* z0 = zeros();
+      Red rr = residual(M, vars, grad.p, res.p);
+      Set need(K, 0);
+      for (int k = 0; k < K; k++) need[k] = M[k] && std::isnan(P[k].theta0);
+      Red th0{};
+      if (any(need)) th0 = reduce(4, need, vars, z0, grad.p);
+      Red kkt = reduce(1, M, vars, z0, grad.p);
+      Red sums = reduce(9, M, vars, z0, nullptr);
+      Red xm = red_slot(1);
+      ipm_ok(amigo_segment_absmax(vars.x, d.xseg, K, ex.dev(xm.off), nullptr, s), "absmax");
+      Red ob{}, lg{};
+      if (o.record_objective) {
+        ob = objective(M, vars.x);
+        lg = reduce(2, M, vars, scalars(per(&Prob::mu)), nullptr);
+      }
+      fetch();
+      const double now = elapsed();
+      for (int k = 0; k < K; k++) {
+        if (!M[k]) continue;
+        Prob& p = P[k];
+        p.res_norm = std::sqrt(at(rr, k, 0));
+        if (need[k]) p.theta0 = at(th0, k);
+        const double du = at(kkt, k, 0), pr = at(kkt, k, 1), co = at(kkt, k, 2);
+        double s_d, s_c;
+        optimality_scaling(at(sums, k, 0), at(sums, k, 1), at(sums, k, 2), at(sums, k, 3), s_d, s_c);
+        const double err = std::max(std::max(du / s_d, pr), co / s_c);
+        int status = AMIGO_IPM_RUNNING;
+        if (err <= o.convergence_tolerance && du <= o.dual_inf_tol && pr <= o.constr_viol_tol &&
+            co <= o.compl_inf_tol) {
+          status = AMIGO_IPM_CONVERGED;
+        } else if (at(xm, k) > o.diverging_iterates_tol) {
+          status = AMIGO_IPM_DIVERGED;
+        } else {
+          const bool acceptable = err <= o.acceptable_tol && du <= o.acceptable_dual_inf_tol &&
+                                  pr <= o.acceptable_constr_viol_tol &&
+                                  co <= o.acceptable_compl_inf_tol;
+          if (o.acceptable_iter > 0 && acceptable) {
+            if (++p.acc_count >= o.acceptable_iter) status = AMIGO_IPM_ACCEPTABLE;
+          } else {
+            p.acc_count = 0;
+          }
+          if (status == AMIGO_IPM_RUNNING) {
+            const double rel = std::fabs(p.res_norm - p.prev_res) / std::max(p.res_norm, 1e-30);
+            p.floor_count = (rel < 1e-14 && p.iter > 0) ? p.floor_count + 1 : 0;
+            if (p.floor_count >= 3 && acceptable) status = AMIGO_IPM_PRECISION_FLOOR;
+          }
+        }
+        amigo_ipm_iteration rec{};
+        rec.iteration = p.iter;
+        rec.time = now;
+        rec.residual = p.res_norm;
+        rec.barrier_param = p.mu;
+        rec.line_iters = p.line_iters;
+        rec.alpha_x = p.rec_ax;
+        rec.x_index = int(p.rec_xi);
+        rec.alpha_z = p.rec_az;
+        rec.z_index = int(p.rec_zi);
+        rec.inertia_delta = p.last_dw;
+        rec.inf_du = du;
+        rec.inf_pr = pr;
+        rec.compl_inf = co;
+        rec.nlp_error = err;
+        rec.filter_size = p.filter.size();
+        rec.objective = o.record_objective ? at(ob, k) + at(lg, k) : kNaN;
+        p.records.push_back(rec);
+        if (status != AMIGO_IPM_RUNNING) {
+          p.status = status;
+          M[k] = 0;
+          stop(k, now);
+        }
+      }
+      for (int k = 0; k < K; k++)
+        if (M[k]) P[k].prev_res = P[k].res_norm;
+    }
+    Set N(K, 0);
+    for (int k = 0; k < K; k++) N[k] = M[k] || RB[k];
+    if (!any(N)) continue;
+    sigma(N, vars, diag.p);
+    copy_x(N, dbase.p, diag.p);
     for (int k = 0; k < K; k++)
-      if (A[k]) P[k].mu_before = P[k].mu;
-    barrier_update(it, A);
+      if (M[k]) P[k].mu_before = P[k].mu;
+    if (any(M)) barrier_update(M);
     for (int k = 0; k < K; k++)
-      if (A[k]) P[k].reg.set_mu(P[k].mu);
+      if (M[k]) P[k].reg.set_mu(P[k].mu);
     toc(PH_CONV, tp);
-    std::vector<char> ok = factorize(A);
+    std::vector<char> ok = factorize(N);
     toc(PH_FACTOR, tp);
     Set B(K, 0);
-    for (int k = 0; k < K; k++) B[k] = A[k] && ok[k];
+    for (int k = 0; k < K; k++) B[k] = N[k] && ok[k];
     if (any(B)) solve_with_mu(B);
     toc(PH_SOLVE, tp);
     Set changed(K, 0);
-    for (int k = 0; k < K; k++) changed[k] = A[k] && P[k].mu != P[k].mu_before;
+    for (int k = 0; k < K; k++) changed[k] = M[k] && P[k].mu != P[k].mu_before;
     if (any(changed)) {
       Red th = reduce(4, changed, vars, zeros(), grad.p);
       fetch();
@@ -1751,7 +1841,7 @@ void amigo_ipm_driver::run() {
       }
     }
     for (int k = 0; k < K; k++) {
-      if (!A[k] || ok[k]) continue;
+      if (!M[k] || ok[k]) continue;
       Prob& p = P[k];
       p.rejections++;
       p.mu = p.mu_before;
@@ -1759,115 +1849,95 @@ void amigo_ipm_driver::run() {
       p.rec_ax = p.rec_az = 0.0;
       p.rec_xi = p.rec_zi = -1;
       increase_barrier(p);
+      p.iter++;
     }
-    if (!any(B)) continue;
-    residual(B, vars, grad.p, res.p);
+    Set BM(K, 0);
+    for (int k = 0; k < K; k++) BM[k] = M[k] && ok[k];
+    if (any(BM)) residual(BM, vars, grad.p, res.p);
+    Set MS(K, 0);
     std::vector<double> tau(K, 1.0);
     for (int k = 0; k < K; k++) {
+      MS[k] = BM[k] || RB[k];
       P[k].tau = tau_of(P[k].mu);
       tau[k] = P[k].tau;
     }
-    Red ms = max_step(B, tau);
-    fetch();
-    std::vector<Search> ls(K);
-    for (int k = 0; k < K; k++) {
-      if (!B[k]) continue;
-      Prob& p = P[k];
-      p.ax = at(ms, k, 0);
-      p.xi = (long long)at(ms, k, 1);
-      p.az = at(ms, k, 2);
-      p.zi = (long long)at(ms, k, 3);
-      if (o.equal_primal_dual_step) p.ax = p.az = std::min(p.ax, p.az);
-      ls[k].alpha_x = p.ax;
-      ls[k].alpha_z = p.az;
-    }
-    line_search_with_watchdog(B, ls);
-    toc(PH_SEARCH, tp);
-    Set rest(K, 0);
-    for (int k = 0; k < K; k++) {
-      if (!B[k]) continue;
-      Prob& p = P[k];
-      p.line_iters = ls[k].iters;
-      if (ls[k].accepted) {
-        p.filter_rejects = ls[k].filter_reject ? p.filter_rejects + 1 : 0;
-        if (p.filter_rejects >= o.filter_reset_trigger && p.filter_resets < o.max_filter_resets) {
-          p.filter.clear();
-          p.filter_resets++;
-          p.filter_rejects = 0;
+    if (any(MS)) {
+      Red ms = max_step(MS, tau);
+      fetch();
+      for (int k = 0; k < K; k++) {
+        if (BM[k]) {
+          Prob& p = P[k];
+          p.ax = at(ms, k, 0);
+          p.xi = (long long)at(ms, k, 1);
+          p.az = at(ms, k, 2);
+          p.zi = (long long)at(ms, k, 3);
+          if (o.equal_primal_dual_step) p.ax = p.az = std::min(p.ax, p.az);
+        } else if (RB[k]) {
+          P[k].rax = at(ms, k, 0);
+          P[k].raz = at(ms, k, 2);
         }
-      } else {
-        rest[k] = 1;
       }
     }
-    Set rejected(K, 0);
-    if (any(rest)) {
-      Set okr = restoration(rest);
+    Set done(K, 0), accepted(K, 0);
+    if (any(BM)) {
+      std::vector<Search> ls(K);
+      for (int k = 0; k < K; k++)
+        if (BM[k]) {
+          ls[k].alpha_x = P[k].ax;
+          ls[k].alpha_z = P[k].az;
+        }
+      line_search_with_watchdog(BM, ls);
+      toc(PH_SEARCH, tp);
+      Set rest(K, 0);
       for (int k = 0; k < K; k++) {
-        if (!rest[k]) continue;
-        P[k].restorations++;
-        if (okr[k]) {
-          ls[k].accepted = true;
+        if (!BM[k]) continue;
+        Prob& p = P[k];
+        p.line_iters = ls[k].iters;
+        p.ls_ratio = ls[k].ratio;
+        if (ls[k].accepted) {
+          p.filter_rejects = ls[k].filter_reject ? p.filter_rejects + 1 : 0;
+          if (p.filter_rejects >= o.filter_reset_trigger && p.filter_resets < o.max_filter_resets) {
+            p.filter.clear();
+            p.filter_resets++;
+            p.filter_rejects = 0;
+          }
+          done[k] = accepted[k] = 1;
+        } else {
+          rest[k] = 1;
+        }
+      }
+      if (any(rest)) {
+        Set quick = restoration_begin(rest);
+        for (int k = 0; k < K; k++) {
+          if (!rest[k]) continue;
+          P[k].restorations++;
+          if (quick[k]) {
+            done[k] = accepted[k] = 1;
+            P[k].line_iters = 0;
+            P[k].wd_shortened = 0;
+          }
+        }
+      }
+    }
+    if (any(RB)) {
+      Set ended = restoration_search(RB);
+      for (int k = 0; k < K; k++) {
+        if (!ended[k]) continue;
+        done[k] = 1;
+        if (P[k].restore_ok) {
+          accepted[k] = 1;
           P[k].line_iters = 0;
           P[k].wd_shortened = 0;
         } else {
-          rejected[k] = 1;
           P[k].rejections++;
         }
       }
     }
     toc(PH_RESTORE, tp);
-    Set G(K, 0);
-    for (int k = 0; k < K; k++) G[k] = B[k] && ls[k].accepted;
-    if (any(G)) {
-      std::vector<double> smin(K, 0.0);
-      for (int k = 0; k < K; k++) {
-        double sm = kEps * std::min(1.0, P[k].mu);
-        if (sm == 0.0) sm = std::numeric_limits<double>::min();
-        smin[k] = sm;
-      }
-      const amigo_ipm_batch BG = batch(mask(G));
-      const amigo_ipm_problem I = info();
-      const amigo_ipm_state S = vars.c();
-      Red fl = red_slot(1);
-      ipm_ok(amigo_ipm_batch_floor_slacks(&BG, &I, &S, d.lbx0, d.ubx0, scalars(per(&Prob::mu)),
-                                          scalars(smin), 1.81898940354586e-12, ws.p,
-                                          ex.dev(fl.off), nullptr, s),
-             "slack floor");
-      ipm_ok(amigo_ipm_batch_reset_bound_multipliers(&BG, &I, &S, scalars(per(&Prob::mu)), 1e10, s),
-             "multiplier reset");
-      gradient(G, vars.x, grad.p);
-    }
-    for (int k = 0; k < K; k++) {
-      if (!B[k]) continue;
-      Prob& p = P[k];
-      if (rejected[k]) {
-        p.rec_ax = p.rec_az = 0.0;
-        p.rec_xi = p.rec_zi = -1;
-        p.mu = p.mu_before;
-        increase_barrier(p);
-      } else {
-        p.rec_ax = ls[k].ratio * p.ax;
-        p.rec_az = p.az;
-        p.rec_xi = p.xi;
-        p.rec_zi = p.zi;
-        p.rejections = 0;
-        if (!p.wd_on) {
-          if (ls[k].ratio == 1.0 || p.line_iters == 1)
-            p.wd_shortened = 0;
-          else
-            p.wd_shortened++;
-        }
-      }
-    }
+    finish_step(done, accepted);
     toc(PH_FINISH, tp);
   }
   hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
-  const double end = elapsed();
-  for (int k = 0; k < K; k++)
-    if (P[k].running) {
-      P[k].t_end = end;
-      if (on_done) on_done(ctx, k);
-    }
 }
 
 // ---- C ABI ----------------------------------------------------------------------
