@@ -115,6 +115,21 @@ __device__ void symm_swap_t(double* F, int ld, int m, int* vars, int a, int b) {
 __device__ void symm_swap(double* F, int ld, int m, int* vars, int a, int b) {
   symm_swap_t<NT>(F, ld, m, vars, a, b);
 }
+// The same interchange on an LDS-resident front (32-bit offsets).
+__device__ __forceinline__ void symm_swap_lds(double* F, int ld, int m, int* vars, int a, int b) {
+  const int t0 = threadIdx.x;
+  __syncthreads();
+  for (int t = t0; t < a; t += NT) swapd(F[a + t * ld], F[b + t * ld]);
+  if (t0 == 0) {
+    swapd(F[a + a * ld], F[b + b * ld]);
+    int v = vars[a];
+    vars[a] = vars[b];
+    vars[b] = v;
+  }
+  for (int t = a + 1 + t0; t < b; t += NT) swapd(F[t + a * ld], F[b + t * ld]);
+  for (int t = b + 1 + t0; t < m; t += NT) swapd(F[t + a * ld], F[t + b * ld]);
+  __syncthreads();
+}
 
 // Swap rows a and b of W for columns [0, ncol).
 template <int NTH>
@@ -437,161 +452,294 @@ __device__ void block_argmax2_dpp(VI& x, VI& y, VI* red) {
   y.i = wy ? __builtin_amdgcn_readlane(c.i, __ffsll((long long)wy) - 1) : INT_MAX;
 }
 
-struct ColStats {
-  double cv;
-  int ci;
-  double bv;
-};
+// ---- wave-local pivot search of an LDS-resident front ---------------------
+// Every wave reads the column it needs from LDS (two rows per lane: fronts
+// of at most LDS_MAX_M = 128 rows) and reduces it with DPP, so all waves
+// hold the same pivot statistics without exchanging them through LDS.
+static_assert(LDS_MAX_M <= 128, "two rows per lane in the wave-local pivot search");
 
-__device__ ColStats block_colstats(const double* F, int ld, int m, int k, int nc,
-                                   double* Wv, VI* red) {
-  // One row per thread (fronts of at most LDS_MAX_M < NT rows), so the
-  // wave reductions can use DPP and a ballot (row order = lane order).
-  VI c = {-1.0, INT_MAX}, b = {-1.0, INT_MAX};
-  const int i = k + 1 + threadIdx.x;
-  if (i < m) {
-    const double w = F[i + (size_t)k * ld];
-    Wv[i] = w;
-    if (i < nc)
-      c = VI{fabs(w), i};
-    else
-      b = VI{fabs(w), i};
-  }
-  block_argmax2_dpp<NT>(c, b, red);
-  return ColStats{c.v, c.i, b.v};
+__device__ __forceinline__ double lane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// LDS-resident front: same pivot rules as factor_unblocked, organized for
-// few barriers.  The common 1x1 step without interchange costs three
-// barriers: column statistics (the column is cached in Wv) and one fused
-// rank-1 update + scaling pass.
-__device__ int factor_front_lds(double* F, int ld, int m, int nfs, int* vars,
-                                bool bk, bool chol, double ustab, double pivot_tol,
-                                double* Wv, VI* red, VI* cred, int* cnt) {
-  const int tid = threadIdx.x;
+// One DPP step of a wave maximum of doubles: lanes without a source keep
+// their own value.
+#define AMIGO_DPP_FMAX(x, ctrl, rm, bm)                                                        \
+  {                                                                                            \
+    const long long b_ = __double_as_longlong(x);                                              \
+    const int lo_ = __builtin_amdgcn_update_dpp((int)b_, (int)b_, ctrl, rm, bm, false);        \
+    const int hi_ = __builtin_amdgcn_update_dpp((int)(b_ >> 32), (int)(b_ >> 32), ctrl, rm, bm, \
+                                                false);                                        \
+    x = fmax(x, __longlong_as_double(((long long)hi_ << 32) | (unsigned)lo_));                 \
+  }
+// Wave maximum of non-negative values (-1 for "none"; no NaN), every lane.
+__device__ __forceinline__ double wave_fmax(double x) {
+  AMIGO_DPP_FMAX(x, 0x111, 0xf, 0xf)  // row_shr:1
+  AMIGO_DPP_FMAX(x, 0x112, 0xf, 0xf)  // row_shr:2
+  AMIGO_DPP_FMAX(x, 0x113, 0xf, 0xf)  // row_shr:3
+  AMIGO_DPP_FMAX(x, 0x114, 0xf, 0xe)  // row_shr:4
+  AMIGO_DPP_FMAX(x, 0x118, 0xf, 0xc)  // row_shr:8
+  AMIGO_DPP_FMAX(x, 0x142, 0xa, 0xf)  // row_bcast:15
+  AMIGO_DPP_FMAX(x, 0x143, 0xc, 0xf)  // row_bcast:31
+  return lane_d(x, 63);
+}
+#undef AMIGO_DPP_FMAX
+
+// |v| as a candidate value: -1 for rows outside [lo, hi) and for NaN
+// (which never wins a pivot search).
+__device__ __forceinline__ double cand_abs(double v, int r, int lo, int hi) {
+  const double a = fabs(v);
+  return (r >= lo && r < hi && a >= 0.0) ? a : -1.0;
+}
+
+// Row of the largest candidate (smallest row on ties) of the two entries of
+// each lane, rows r0 = base + lane and r1 = r0 + 64; mx is the wave maximum
+// of those entries (>= 0).  Every row of the first half precedes every row
+// of the second, so the first half is asked first.
+__device__ __forceinline__ int wave_argmax_row(double c0, double c1, double mx, int r0, int r1) {
+  const unsigned long long b0 = __ballot(c0 == mx), b1 = __ballot(c1 == mx);
+  if (b0) return __builtin_amdgcn_readlane(r0, __ffsll((long long)b0) - 1);
+  return __builtin_amdgcn_readlane(r1, __ffsll((long long)b1) - 1);
+}
+
+// LDS-resident front (m <= LDS_MAX_M): the pivot rules of factor_unblocked
+// with one barrier per common step.  Lane l of every wave holds rows
+// r = k + l (and r + 64 when ROWS = 2, fronts of more than 64 rows) of the
+// pivot column; the largest entry below the diagonal is one DPP reduction,
+// and only when the 1x1 test fails are the candidate row imax and row /
+// column imax searched.  The decision is then the same in all waves.  The
+// trailing update reads the pivot block's unscaled columns (its multipliers
+// come from the lanes by readlane), so the columns are scaled to L in the
+// next step's update pass by wave 0, whose lanes hold exactly the rows to
+// scale (nothing reads those columns in between; the symmetric
+// interchanges commute with the scaling, which is the same for every row),
+// and after the last step.  Loads are clamped into the front so that they
+// need no exec mask; a lane stores its row's entries of the trailing
+// columns whenever the row lies in the front, entries above the diagonal
+// included: the upper triangle of an LDS front is never read (the
+// interchanges and the pivot search touch the lower triangle only) and is
+// written back as zeros.
+template <int ROWS>
+__device__ __forceinline__ int factor_front_lds(double* F, int ld, int m, int nfs, int* vars,
+                                                bool bk, bool chol, double ustab,
+                                                double pivot_tol, int* cnt) {
+  static_assert(ROWS == 1 || ROWS == 2, "one or two rows per lane");
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  constexpr int NW = NT / 64;
+  const int mlast = m - 1;
   int k = 0, nc = nfs;
+  // The pivot block whose columns still hold the unscaled values: its rows
+  // to scale start at the current k.
+  int pk = -1, pstep = 0;
+  double q0 = 0.0, q1 = 0.0, q2 = 0.0;  // 1x1: 1/d; 2x2: i00, i01, i11
+  __syncthreads();
   while (k < nc) {
     int step = 1, kpiv = k;
-    bool zero = false;
-    const ColStats st = block_colstats(F, ld, m, k, nc, Wv, cred);
+    bool zero = false, read_imax = false;
+    const int r0 = k + lane, r1 = r0 + 64;
+    const int c0 = min(r0, mlast), c1 = min(r1, mlast);
+    // Column k from the diagonal down, rows r0 (and r1) of this lane.
+    double v0 = F[c0 + k * ld], v1 = ROWS == 2 ? F[c1 + k * ld] : 0.0;
     if (!chol) {
-      const double akk = fabs(F[k + (size_t)k * ld]);
-      const bool has_cand = k + 1 < nc && st.ci > k && st.ci < nc;
-      const int imax = has_cand ? st.ci : 0;
-      const double abswik = has_cand ? st.cv : 0.0;
-      const double colmax = bk ? abswik : fmax(abswik, nc < m ? st.bv : 0.0);
-      const bool ok1 = bk ? (fmax(akk, colmax) == 0.0 || akk >= BK_ALPHA * colmax)
-                          : (akk >= pivot_tol && akk >= ustab * colmax);
+      const double akk = fabs(F[k + k * ld]);
+      const double a0 = cand_abs(v0, r0, k + 1, m);
+      const double a1 = ROWS == 2 ? cand_abs(v1, r1, k + 1, m) : -1.0;
+      // Largest entry below the diagonal: the 1x1 test's column maximum
+      // (fronts without contribution rows have candidates only).
+      const double allmax = wave_fmax(fmax(a0, a1));
+      const double colmax0 = fmax(allmax, 0.0);
+      const bool ok1 = bk ? (fmax(akk, colmax0) == 0.0 || akk >= BK_ALPHA * colmax0)
+                          : (akk >= pivot_tol && akk >= ustab * colmax0);
       if (ok1) {
-        zero = bk && fmax(akk, colmax) == 0.0;
-      } else if (has_cand) {
-        VI r = {-1.0, INT_MAX}, dummy = {-1.0, INT_MAX};
-        const int i = k + tid;
-        if (i < m && i != imax) {
-          const double g = i < imax ? F[imax + (size_t)i * ld] : F[i + (size_t)imax * ld];
-          r = VI{fabs(g), i};
-        }
-        block_argmax2_dpp<NT>(r, dummy, red);
-        const double rowmax = r.v;
-        const double aii = fabs(F[imax + (size_t)imax * ld]);
-        if (bk) {
-          if (akk >= BK_ALPHA * colmax * (colmax / rowmax)) {
-            step = 1;
-            kpiv = k;
-          } else if (aii >= BK_ALPHA * rowmax) {
-            step = 1;
-            kpiv = imax;
-          } else {
-            step = 2;
-            kpiv = imax;
-          }
-        } else if (aii >= pivot_tol && aii >= ustab * rowmax) {
-          step = 1;
-          kpiv = imax;
-        } else {
-          const double wkk = F[k + (size_t)k * ld];
-          const double wik = F[imax + (size_t)k * ld];
-          const double wii = F[imax + (size_t)imax * ld];
-          const double det = fabs(wkk * wii - wik * wik);
-          step = 0;
-          if (det >= pivot_tol) {
-            const double delta = det / fmax(aii + abswik, akk + abswik);
-            if (delta >= ustab * colmax && delta >= ustab * rowmax) {
+        zero = bk && fmax(akk, colmax0) == 0.0;
+      } else {
+        // Candidates: rows k + 1 .. nc - 1.
+        const double e0 = cand_abs(v0, r0, k + 1, nc);
+        const double e1 = ROWS == 2 ? cand_abs(v1, r1, k + 1, nc) : -1.0;
+        const double cmx = k + 1 < nc ? wave_fmax(fmax(e0, e1)) : -1.0;
+        if (cmx >= 0.0) {
+          read_imax = true;
+          const int imax = wave_argmax_row(e0, e1, cmx, r0, r1);
+          const double abswik = cmx;
+          const double colmax = colmax0;
+          // Row / column imax from row k down, without the diagonal.
+          auto g = [&](int i, int ic) {
+            const double x = ic < imax ? F[imax + ic * ld] : F[ic + imax * ld];
+            const double a = fabs(x);
+            return (i < m && i != imax && a >= 0.0) ? a : -1.0;
+          };
+          const double g0 = g(r0, c0), g1 = ROWS == 2 ? g(r1, c1) : -1.0;
+          const double rowmax = wave_fmax(fmax(g0, g1));
+          const double aii = fabs(F[imax + imax * ld]);
+          if (bk) {
+            if (akk >= BK_ALPHA * colmax * (colmax / rowmax)) {
+              step = 1;
+              kpiv = k;
+            } else if (aii >= BK_ALPHA * rowmax) {
+              step = 1;
+              kpiv = imax;
+            } else {
               step = 2;
               kpiv = imax;
             }
+          } else if (aii >= pivot_tol && aii >= ustab * rowmax) {
+            step = 1;
+            kpiv = imax;
+          } else {
+            const double wkk = F[k + k * ld];
+            const double wik = F[imax + k * ld];
+            const double wii = F[imax + imax * ld];
+            const double det = fabs(wkk * wii - wik * wik);
+            step = 0;
+            if (det >= pivot_tol) {
+              const double delta = det / fmax(aii + abswik, akk + abswik);
+              if (delta >= ustab * colmax && delta >= ustab * rowmax) {
+                step = 2;
+                kpiv = imax;
+              }
+            }
           }
+        } else {
+          step = 0;
         }
-      } else {
-        step = 0;
       }
     }
     if (step == 0) {
-      if (k < nc - 1) symm_swap(F, ld, m, vars, k, nc - 1);
+      // symm_swap_lds starts with a barrier: every wave has read what it moves.
+      if (k < nc - 1) symm_swap_lds(F, ld, m, vars, k, nc - 1);
       nc -= 1;
       continue;
     }
     const int ksrc = k + step - 1;
-    const bool swapped = kpiv != ksrc;
-    if (swapped) symm_swap(F, ld, m, vars, ksrc, kpiv);
-    if (step == 1) {
-      const double d = F[k + (size_t)k * ld];
-      if (swapped) {
-        for (int i = k + 1 + tid; i < m; i += NT) Wv[i] = F[i + (size_t)k * ld];
-        __syncthreads();
-      }
-      if (!zero) {
-        // Rank-1 update of the lower trailing triangle: a wave per column,
-        // lanes down the rows.
-        const double dinv = 1.0 / d;
-        const int lane = tid & 63, wv = tid >> 6;
-        for (int j = k + 1 + wv; j < m; j += NT / 64) {
-          const double wj = Wv[j];
-          double* Fj = F + (size_t)j * ld;
-          for (int i = j + lane; i < m; i += 64) Fj[i] -= (Wv[i] * dinv) * wj;
+    if (kpiv != ksrc) {
+      symm_swap_lds(F, ld, m, vars, ksrc, kpiv);
+      v0 = F[c0 + k * ld];
+      if (ROWS == 2) v1 = F[c1 + k * ld];
+    } else if (read_imax) {
+      __syncthreads();  // row imax is read above and updated below
+    }
+    // The previous block's columns to L (wave 0: its lanes hold rows k..).
+    if (pk >= 0 && wv == 0) {
+      if (pstep == 1) {
+        if (r0 < m) F[r0 + pk * ld] *= q0;
+        if (ROWS == 2 && r1 < m) F[r1 + pk * ld] *= q0;
+      } else {
+        if (r0 < m) {
+          const double w1 = F[r0 + pk * ld], w2 = F[r0 + (pk + 1) * ld];
+          F[r0 + pk * ld] = w1 * q0 + w2 * q1;
+          F[r0 + (pk + 1) * ld] = w1 * q1 + w2 * q2;
         }
-        for (int i = k + 1 + tid; i < m; i += NT) F[i + (size_t)k * ld] = Wv[i] * dinv;
+        if (ROWS == 2 && r1 < m) {
+          const double w1 = F[r1 + pk * ld], w2 = F[r1 + (pk + 1) * ld];
+          F[r1 + pk * ld] = w1 * q0 + w2 * q1;
+          F[r1 + (pk + 1) * ld] = w1 * q1 + w2 * q2;
+        }
+      }
+    }
+    pk = -1;
+    // The trailing lower triangle from this block's unscaled columns: a
+    // wave per column, lanes down the rows, the loads of UC columns in
+    // flight together.
+    constexpr int UC = 8;
+    // Value of a lane-held column at row j >= k (uniform j).
+    auto at_row = [&](double x0, double x1, int j) {
+      const int t = j - k;
+      return lane_d(ROWS == 2 && t >= 64 ? x1 : x0, t & 63);
+    };
+    if (step == 1) {
+      const double d = F[k + k * ld];
+      if (!zero) {
+        const double dinv = 1.0 / d;
+        const double l0 = v0 * dinv, l1 = v1 * dinv;
+        for (int jb = k + 1 + wv; jb < m; jb += NW * UC) {
+          double cj[UC], f0[UC], f1[UC];
+#pragma unroll
+          for (int u = 0; u < UC; u++) {
+            const int j = min(jb + u * NW, mlast);
+            cj[u] = at_row(v0, v1, j);
+            f0[u] = F[c0 + j * ld];
+            f1[u] = ROWS == 2 ? F[c1 + j * ld] : 0.0;
+          }
+          if (r0 < m) {
+#pragma unroll
+            for (int u = 0; u < UC; u++)
+              if (jb + u * NW < m) F[r0 + (jb + u * NW) * ld] = f0[u] - l0 * cj[u];
+          }
+          if (ROWS == 2 && r1 < m) {
+#pragma unroll
+            for (int u = 0; u < UC; u++)
+              if (jb + u * NW < m) F[r1 + (jb + u * NW) * ld] = f1[u] - l1 * cj[u];
+          }
+        }
+        pk = k;
+        pstep = 1;
+        q0 = dinv;
       }
       if (tid == 0) count_inertia(1, d, 0.0, 0.0, zero, cnt);
-      __syncthreads();
       k += 1;
     } else {
-      const double a = F[k + (size_t)k * ld];
-      const double b = F[k + 1 + (size_t)k * ld];
-      const double c = F[k + 1 + (size_t)(k + 1) * ld];
+      const double a = F[k + k * ld];
+      const double b = F[k + 1 + k * ld];
+      const double c = F[k + 1 + (k + 1) * ld];
       const double detinv = 1.0 / (a * c - b * b);
       const double i00 = c * detinv, i01 = -b * detinv, i11 = a * detinv;
-      // L columns of the 2x2 block first (Wv, Wv2), then the rank-2
-      // update of the lower trailing triangle, a wave per column.
-      double* Wv2 = Wv + LDS_MAX_M;
-      for (int i = k + 2 + tid; i < m; i += NT) {
-        const double w1 = F[i + (size_t)k * ld], w2 = F[i + (size_t)(k + 1) * ld];
-        Wv[i] = w1 * i00 + w2 * i01;
-        Wv2[i] = w1 * i01 + w2 * i11;
-      }
-      __syncthreads();
-      {
-        const int lane = tid & 63, wv = tid >> 6;
-        for (int j = k + 2 + wv; j < m; j += NT / 64) {
-          const double a1 = F[j + (size_t)k * ld], a2 = F[j + (size_t)(k + 1) * ld];
-          double* Fj = F + (size_t)j * ld;
-          for (int i = j + lane; i < m; i += 64) Fj[i] -= Wv[i] * a1 + Wv2[i] * a2;
+      // Column k + 1 at this lane's rows; L of the block at rows >= k + 2.
+      const double u0 = F[c0 + (k + 1) * ld], u1 = ROWS == 2 ? F[c1 + (k + 1) * ld] : 0.0;
+      const double p0 = v0 * i00 + u0 * i01, s0 = v0 * i01 + u0 * i11;
+      const double p1 = v1 * i00 + u1 * i01, s1 = v1 * i01 + u1 * i11;
+      for (int jb = k + 2 + wv; jb < m; jb += NW * UC) {
+        double a1[UC], a2[UC], f0[UC], f1[UC];
+#pragma unroll
+        for (int u = 0; u < UC; u++) {
+          const int j = min(jb + u * NW, mlast);
+          a1[u] = at_row(v0, v1, j);
+          a2[u] = at_row(u0, u1, j);
+          f0[u] = F[c0 + j * ld];
+          f1[u] = ROWS == 2 ? F[c1 + j * ld] : 0.0;
+        }
+        if (r0 < m) {
+#pragma unroll
+          for (int u = 0; u < UC; u++)
+            if (jb + u * NW < m)
+              F[r0 + (jb + u * NW) * ld] = f0[u] - (p0 * a1[u] + s0 * a2[u]);
+        }
+        if (ROWS == 2 && r1 < m) {
+#pragma unroll
+          for (int u = 0; u < UC; u++)
+            if (jb + u * NW < m)
+              F[r1 + (jb + u * NW) * ld] = f1[u] - (p1 * a1[u] + s1 * a2[u]);
         }
       }
-      __syncthreads();
-      for (int i = k + 2 + tid; i < m; i += NT) {
-        F[i + (size_t)k * ld] = Wv[i];
-        F[i + (size_t)(k + 1) * ld] = Wv2[i];
-      }
+      pk = k;
+      pstep = 2;
+      q0 = i00;
+      q1 = i01;
+      q2 = i11;
       if (tid == 0) {
         vars[k] = -(vars[k] + 1);
         vars[k + 1] = -(vars[k + 1] + 1);
         count_inertia(2, a, b, c, false, cnt);
       }
-      __syncthreads();
       k += 2;
     }
+    __syncthreads();
   }
+  // The last block's columns: its rows to scale are those from k down.
+  if (pk >= 0 && wv == 0) {
+    for (int r = k + lane; r < m; r += 64) {
+      if (pstep == 1) {
+        F[r + pk * ld] *= q0;
+      } else {
+        const double w1 = F[r + pk * ld], w2 = F[r + (pk + 1) * ld];
+        F[r + pk * ld] = w1 * q0 + w2 * q1;
+        F[r + (pk + 1) * ld] = w1 * q1 + w2 * q2;
+      }
+    }
+  }
+  __syncthreads();
   return k;
 }
 
@@ -701,7 +849,6 @@ __global__ __launch_bounds__(NT) void level_kernel(
   const bool big = m > LDS_MAX_M;
   const long long cap = node_cap ? (long long)node_cap[node] : (long long)lds_doubles;
   const bool in_lds = !big && (long long)m * m <= cap;
-  double* F = in_lds ? smem : G;
   const int ld = m;
 
   // Front variables: pivots of the supernode, delayed pivots of the
@@ -749,93 +896,110 @@ __global__ __launch_bounds__(NT) void level_kernel(
     }
     return;
   }
-  const size_t mm = (size_t)m * m;
-  for (size_t e = tid; e < mm; e += NT) F[e] = 0.0;
-  __syncthreads();
+  // Assembly into the front, instantiated for the LDS front (smem itself,
+  // so that the accesses compile to LDS instructions) and for a front in
+  // global memory.
+  auto assemble = [&](double* F) __attribute__((always_inline)) {
+    const size_t mm = (size_t)m * m;
+    for (size_t e = tid; e < mm; e += NT) F[e] = 0.0;
+    __syncthreads();
 
-  // Original entries of the supernode's columns.
-  for (int t = P.aptr[node] + tid; t < P.aptr[node + 1]; t += NT) {
-    int r = P.arow[t];
-    r = r < ns ? r : r + D;
-    const int c = P.acol[t];
-    double v = vals[P.aidx[t]];
-    if (sc) v *= sc[vars[r]] * sc[vars[c]];
-    if (r >= c)
-      F[r + (size_t)c * ld] += v;
-    else
-      F[c + (size_t)r * ld] += v;
-  }
-  __syncthreads();
-  if (shift != nullptr) {
-    for (int j = tid; j < ns; j += NT) {
-      const int v = vars[j];
-      F[j + (size_t)j * ld] += sc ? shift[v] * sc[v] * sc[v] : shift[v];
+    // Original entries of the supernode's columns.
+    for (int t = P.aptr[node] + tid; t < P.aptr[node + 1]; t += NT) {
+      int r = P.arow[t];
+      r = r < ns ? r : r + D;
+      const int c = P.acol[t];
+      double v = vals[P.aidx[t]];
+      if (sc) v *= sc[vars[r]] * sc[vars[c]];
+      if (r >= c)
+        F[r + (size_t)c * ld] += v;
+      else
+        F[c + (size_t)r * ld] += v;
     }
     __syncthreads();
-  }
+    if (shift != nullptr) {
+      for (int j = tid; j < ns; j += NT) {
+        const int v = vars[j];
+        F[j + (size_t)j * ld] += sc ? shift[v] * sc[v] * sc[v] : shift[v];
+      }
+      __syncthreads();
+    }
 
-  // Extend-add of the children's contribution blocks, one child at a time
-  // (child order: deterministic sums).  The child's row positions in this
-  // front go to LDS first (a child's update rows all lie in this front, so
-  // there are at most m <= LDS_MAX_M of them); then each thread has 16 loads
-  // of the contribution block in flight.
-  {
-    int doff = ns;
-    for (int q0 = kb; q0 < ke; q0 += KID_BATCH) {
-      const int nb = min(KID_BATCH, ke - q0);
-      if (!one_batch || D == 0) load_kids(q0, nb);
-      for (int qq = 0; qq < nb; qq++) {
-        const int pc = km[qq].pc, dc = km[qq].dc, mc = km[qq].mc;
-        const int w = mc - pc;
-        const int* cm = P.cmap + km[qq].cptr - dc;  // cm[ii] for ii >= dc
-        for (int ii = tid; ii < w; ii += NT) {
-          int pos;
-          if (ii < dc) {
-            pos = doff + ii;
-          } else {
-            const int sidx = cm[ii];
-            pos = sidx < ns ? sidx : sidx + D;
+    // Extend-add of the children's contribution blocks, one child at a time
+    // (child order: deterministic sums).  The child's row positions in this
+    // front go to LDS first (a child's update rows all lie in this front, so
+    // there are at most m <= LDS_MAX_M of them); then each thread has 16 loads
+    // of the contribution block in flight.
+    {
+      int doff = ns;
+      for (int q0 = kb; q0 < ke; q0 += KID_BATCH) {
+        const int nb = min(KID_BATCH, ke - q0);
+        if (!one_batch || D == 0) load_kids(q0, nb);
+        for (int qq = 0; qq < nb; qq++) {
+          const int pc = km[qq].pc, dc = km[qq].dc, mc = km[qq].mc;
+          const int w = mc - pc;
+          const int* cm = P.cmap + km[qq].cptr - dc;  // cm[ii] for ii >= dc
+          for (int ii = tid; ii < w; ii += NT) {
+            int pos;
+            if (ii < dc) {
+              pos = doff + ii;
+            } else {
+              const int sidx = cm[ii];
+              pos = sidx < ns ? sidx : sidx + D;
+            }
+            rowpos[ii] = pos;
           }
-          rowpos[ii] = pos;
-        }
-        __syncthreads();
-        const double* Cf = S.rpool + km[qq].foff + pc + (size_t)pc * mc;
-        const int tot = w * w;
-        for (int e0 = tid; e0 < tot; e0 += 16 * NT) {
-          double a[16];
-#pragma unroll
-          for (int u = 0; u < 16; u++) {
-            const int e = e0 + u * NT, jj = e / w, ii = e - jj * w;
-            a[u] = (e < tot && ii >= jj) ? Cf[ii + (size_t)jj * mc] : 0.0;
-          }
-#pragma unroll
-          for (int u = 0; u < 16; u++) {
-            const int e = e0 + u * NT, jj = e / w, ii = e - jj * w;
-            if (e < tot && ii >= jj) {
-              const int pi = rowpos[ii], pj = rowpos[jj];
-              if (pi >= pj)
-                F[pi + (size_t)pj * ld] += a[u];
-              else
-                F[pj + (size_t)pi * ld] += a[u];
+          __syncthreads();
+          const double* Cf = S.rpool + km[qq].foff + pc + (size_t)pc * mc;
+          const int tot = w * w;
+          for (int e0 = tid; e0 < tot; e0 += 16 * NT) {
+            double a[16];
+  #pragma unroll
+            for (int u = 0; u < 16; u++) {
+              const int e = e0 + u * NT, jj = e / w, ii = e - jj * w;
+              a[u] = (e < tot && ii >= jj) ? Cf[ii + (size_t)jj * mc] : 0.0;
+            }
+  #pragma unroll
+            for (int u = 0; u < 16; u++) {
+              const int e = e0 + u * NT, jj = e / w, ii = e - jj * w;
+              if (e < tot && ii >= jj) {
+                const int pi = rowpos[ii], pj = rowpos[jj];
+                if (pi >= pj)
+                  F[pi + (size_t)pj * ld] += a[u];
+                else
+                  F[pj + (size_t)pi * ld] += a[u];
+              }
             }
           }
+          doff += dc;
+          __syncthreads();
         }
-        doff += dc;
-        __syncthreads();
       }
     }
-  }
+  };
+  if (in_lds)
+    assemble(smem);
+  else
+    assemble(G);
 
+  // The LDS front is passed as smem itself, so that its accesses compile to
+  // LDS instructions rather than generic (flat) ones.
   int np;
   if (in_lds) {
-    np = factor_front_lds(F, ld, m, ns + D, vars, cs == 0, chol != 0, ustab,
-                          pivot_tol, smem + lds_doubles, red, cred, cnt);
+    if (m <= 64)
+      np = factor_front_lds<1>(smem, m, m, ns + D, vars, cs == 0, chol != 0, ustab, pivot_tol,
+                               cnt);
+    else
+      np = factor_front_lds<2>(smem, m, m, ns + D, vars, cs == 0, chol != 0, ustab, pivot_tol,
+                               cnt);
+    // The lower triangle and zeros above it.
+    for (int e = tid; e < m * m; e += NT) {
+      const int j = e / m, i = e - j * m;
+      G[e] = i >= j ? smem[e] : 0.0;
+    }
   } else {
-    np = factor_unblocked(F, ld, m, ns + D, vars, cs == 0, chol != 0, ustab,
+    np = factor_unblocked(G, ld, m, ns + D, vars, cs == 0, chol != 0, ustab,
                           pivot_tol, red, cnt);
-  }
-  if (in_lds) {
-    for (size_t e = tid; e < mm; e += NT) G[e] = F[e];
   }
   if (tid == 0) {
     S.npiv[node] = np;
