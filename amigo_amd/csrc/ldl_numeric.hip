@@ -1133,24 +1133,13 @@ constexpr size_t PANEL_LDS_BYTES = sizeof(double) * LS_COLS * NTP;
 
 struct PanelShared {
   VI red[2 * NTP / 64];
-  double wrow[NBR + 2];
+  double wmax[2][NTP / 64];  // per-wave column maxima (double-buffered)
+  double wrow[2][NBR + 2];   // row k of W (double-buffered)
   double wrow2[NBR + 2];
-  double dd[NBR + 2];   // D diagonal of panel column q
-  double de[NBR + 2];   // D sub-diagonal of a 2x2 block starting at q
-  int two[NBR + 2];     // 0: 1x1, 1: first column of a 2x2, 2: second
   double xch[4];
-  double sval[4];
+  double sval[2][4];      // pivot-row values (double-buffered)
   int cnt[4];
 };
-
-// Entry q of row t of W = L D (1x1 and 2x2 blocks of D).
-template <int NTH>
-__device__ __forceinline__ double ld_entry(const double* Ls, int t, int q, const PanelShared& sh) {
-  const double l = Ls[q * NTH + t];
-  if (sh.two[q] == 0) return l * sh.dd[q];
-  if (sh.two[q] == 1) return l * sh.dd[q] + Ls[(q + 1) * NTH + t] * sh.de[q];
-  return Ls[(q - 1) * NTH + t] * sh.de[q - 1] + l * sh.dd[q];
-}
 
 // s - sum_{q < kw} Ls[q][t] w[q], kw <= LS_COLS.  Fully unrolled with the
 // bound as a predicate, so all LDS reads are issued before the first FMA
@@ -1189,6 +1178,21 @@ __device__ __forceinline__ void swap_lds_rows_values(double& s, double& s2, int 
     s2 = sh.xch[1];
   }
   __syncthreads();
+}
+
+// Block-wide maximum of non-negative values (-1: none), every thread; buf
+// selects the double buffer.  One barrier.
+template <int NTH>
+__device__ __forceinline__ double block_fmax(double x, PanelShared& sh, int buf) {
+  constexpr int NW = NTH / 64;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const double w = wave_fmax(x);
+  if (lane == 0) sh.wmax[buf][wid] = w;
+  __syncthreads();
+  double r = sh.wmax[buf][0];
+#pragma unroll
+  for (int q = 1; q < NW; q++) r = fmax(r, sh.wmax[buf][q]);
+  return r;
 }
 
 // Panel block <-> global front: columns [kstart, kstart + pw), rows >= the
@@ -1265,6 +1269,17 @@ __device__ void panel_symm_swap(double* F, int ld, int m, int kstart, int* vars,
   __syncthreads();
 }
 
+// One LDS panel (NBR pivots) of a blocked front.  Column k is updated with
+// row k of W (the unscaled updated columns, W[k, :kw], as the reference's
+// panel loop uses it), read from global memory by the lanes of the last
+// wave.  In the common step (a 1x1 pivot without interchange) the next
+// pivot row is k + 1, so its W row is published for the next column before
+// the step's one barrier (its earlier entries read while the column's dot
+// products run, this column's entry the row's own value), and that barrier
+// also carries the column maximum (a value-only reduction) and the pivot
+// row's value; any other step publishes the next W row after an extra
+// barrier.  Shared values used across that single barrier are
+// double-buffered.
 template <int NTH>
 __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& nc, int& kw,
                           int kstart, bool bk, bool chol, double ustab, double pivot_tol,
@@ -1274,6 +1289,8 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
   const int r = kstart + tid;
   const bool own = r < m;
   const int pw = min(LS_COLS, m - kstart);
+  // Lane q of the last wave reads W[row, q] of a row for the W-row buffers.
+  const int wlane = tid - (NTH - 64);
   panel_load<true, NTH>(F, ld, m, kstart, pw, Ls);
   __syncthreads();
   // Interchange of a < b, in LDS when both columns are in the panel,
@@ -1290,98 +1307,119 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
     swap_w_rows_t<NTH>(W, ldw, a, b, wcols);
     swap_lds_rows_values(s, s2, tid, a - kstart, b - kstart, sh);
   };
+  // Publish row k of W into wrow[buf] (the rows' entries are in W).
+  auto publish = [&](int buf) {
+    if (wlane >= 0 && wlane < kw && k < m) sh.wrow[buf][wlane] = W[k + (size_t)wlane * ldw];
+  };
+  int wb = 0;
+  // The first pivot row's W row is empty (kw = 0 at the panel's start).
   while (kw < NBR && k < nc) {
-    // Row k of W = L D, one entry per thread.
-    if (tid < kw) sh.wrow[tid] = ld_entry<NTH>(Ls, k - kstart, tid, sh);
-    __syncthreads();
+    const double* wr = sh.wrow[wb];
     double s = 0.0, s2 = 0.0;
-    VI a = {-1.0, INT_MAX}, b = {-1.0, INT_MAX};
+    double a_all = -1.0;
+    // Row k + 1 of W over the earlier columns, in flight during the dots.
+    const bool wnext = wlane >= 0 && wlane < kw && k + 1 < m;
+    const double wn = wnext ? W[k + 1 + (size_t)wlane * ldw] : 0.0;
     if (own && r >= k) {
-      s = lds_dot<NTH>(Ls[kw * NTH + tid], Ls, tid, sh.wrow, kw);
+      s = lds_dot<NTH>(Ls[kw * NTH + tid], Ls, tid, wr, kw);
       W[r + (size_t)kw * ldw] = s;
-      if (r == k) sh.sval[0] = s;
-      if (r > k) {
-        const VI c = {fabs(s), r};
-        if (r < nc)
-          a = c;
-        else
-          b = c;
+      if (r == k) {
+        sh.sval[wb][0] = s;
+        sh.sval[wb][3] = 1.0 / s;  // the 1x1 pivot's reciprocal, off the others' path
       }
+      if (r > k) {
+        const double a = fabs(s);
+        a_all = a >= 0.0 ? a : -1.0;
+      }
+      // The next column's W row if this step is a 1x1 pivot without
+      // interchange: row k + 1, this column's entry included.
+      if (r == k + 1) sh.wrow[wb ^ 1][kw] = s;
     }
+    if (wnext) sh.wrow[wb ^ 1][wlane] = wn;
     int step = 1, kpiv = k;
-    bool zero = false;
+    bool zero = false, plain = true;
+    double colmax0 = 0.0, akk = 0.0;
     if (chol) {
       __syncthreads();
     } else {
-      block_argmax2_dpp<NTH>(a, b, sh.red);
-      const double akk = fabs(sh.sval[0]);
-      const bool has_cand = k + 1 < nc && a.i > k && a.i < nc;
-      const int imax = has_cand ? a.i : 0;
-      const double abswik = has_cand ? a.v : 0.0;
-      const double colmax = bk ? abswik : fmax(abswik, nc < m ? b.v : 0.0);
-      const bool ok1 = bk ? (fmax(akk, colmax) == 0.0 || akk >= BK_ALPHA * colmax)
-                          : (akk >= pivot_tol && akk >= ustab * colmax);
+      // Largest entry below the diagonal (candidates and contribution rows):
+      // the 1x1 test's column maximum.
+      colmax0 = fmax(block_fmax<NTH>(a_all, sh, wb), 0.0);
+      akk = fabs(sh.sval[wb][0]);
+      const bool ok1 = bk ? (fmax(akk, colmax0) == 0.0 || akk >= BK_ALPHA * colmax0)
+                          : (akk >= pivot_tol && akk >= ustab * colmax0);
       if (ok1) {
-        zero = bk && fmax(akk, colmax) == 0.0;
-      } else if (has_cand) {
-        if (tid < kw) sh.wrow2[tid] = ld_entry<NTH>(Ls, imax - kstart, tid, sh);
-        // Not yet updated column imax, rows >= k: rows below imax from its
-        // column, rows k..imax-1 from row imax (the columns r < imax).
-        double fi = 0.0;
-        if (own && r >= k) {
-          const int qi = imax - kstart;
-          if (r >= imax)
-            fi = qi < pw ? Ls[qi * NTH + tid] : F[r + (size_t)imax * ld];
-          else
-            fi = tid < pw ? Ls[tid * NTH + qi] : F[imax + (size_t)r * ld];
-        }
-        __syncthreads();
-        VI rr = {-1.0, INT_MAX}, dummy = {-1.0, INT_MAX};
-        if (own && r >= k) {
-          s2 = lds_dot<NTH>(fi, Ls, tid, sh.wrow2, kw);
-          W[r + (size_t)(kw + 1) * ldw] = s2;
-          if (r == imax) {
-            sh.sval[1] = s2;
-            sh.sval[2] = s;
-          } else {
-            rr = VI{fabs(s2), r};
+        zero = bk && fmax(akk, colmax0) == 0.0;
+      } else {
+        plain = false;
+        // The candidate with the largest entry (smallest row on ties).
+        VI a = {-1.0, INT_MAX}, b = {-1.0, INT_MAX};
+        if (own && r > k && r < nc) a = VI{fabs(s), r};
+        block_argmax2_dpp<NTH>(a, b, sh.red);
+        const bool has_cand = k + 1 < nc && a.i > k && a.i < nc;
+        const int imax = has_cand ? a.i : 0;
+        const double abswik = has_cand ? a.v : 0.0;
+        const double colmax = colmax0;
+        if (has_cand) {
+          if (wlane >= 0 && wlane < kw) sh.wrow2[wlane] = W[imax + (size_t)wlane * ldw];
+          // Not yet updated column imax, rows >= k: rows below imax from its
+          // column, rows k..imax-1 from row imax (the columns r < imax).
+          double fi = 0.0;
+          if (own && r >= k) {
+            const int qi = imax - kstart;
+            if (r >= imax)
+              fi = qi < pw ? Ls[qi * NTH + tid] : F[r + (size_t)imax * ld];
+            else
+              fi = tid < pw ? Ls[tid * NTH + qi] : F[imax + (size_t)r * ld];
           }
-        }
-        block_argmax2_dpp<NTH>(rr, dummy, sh.red);
-        const double rowmax = rr.v;
-        const double aii = fabs(sh.sval[1]);
-        if (bk) {
-          if (akk >= BK_ALPHA * colmax * (colmax / rowmax)) {
-            step = 1;
-            kpiv = k;
-          } else if (aii >= BK_ALPHA * rowmax) {
-            step = 1;
-            kpiv = imax;
-          } else {
-            step = 2;
-            kpiv = imax;
+          __syncthreads();
+          VI rr = {-1.0, INT_MAX}, dummy = {-1.0, INT_MAX};
+          if (own && r >= k) {
+            s2 = lds_dot<NTH>(fi, Ls, tid, sh.wrow2, kw);
+            W[r + (size_t)(kw + 1) * ldw] = s2;
+            if (r == imax) {
+              sh.sval[wb][1] = s2;
+              sh.sval[wb][2] = s;
+            } else {
+              rr = VI{fabs(s2), r};
+            }
           }
-        } else if (aii >= pivot_tol && aii >= ustab * rowmax) {
-          step = 1;
-          kpiv = imax;
-        } else {
-          const double wkk = sh.sval[0], wii = sh.sval[1], wik = sh.sval[2];
-          const double det = fabs(wkk * wii - wik * wik);
-          step = 0;
-          if (det >= pivot_tol) {
-            const double delta = det / fmax(aii + abswik, akk + abswik);
-            if (delta >= ustab * colmax && delta >= ustab * rowmax) {
+          block_argmax2_dpp<NTH>(rr, dummy, sh.red);
+          const double rowmax = rr.v;
+          const double aii = fabs(sh.sval[wb][1]);
+          if (bk) {
+            if (akk >= BK_ALPHA * colmax * (colmax / rowmax)) {
+              step = 1;
+              kpiv = k;
+            } else if (aii >= BK_ALPHA * rowmax) {
+              step = 1;
+              kpiv = imax;
+            } else {
               step = 2;
               kpiv = imax;
             }
+          } else if (aii >= pivot_tol && aii >= ustab * rowmax) {
+            step = 1;
+            kpiv = imax;
+          } else {
+            const double wkk = sh.sval[wb][0], wii = sh.sval[wb][1], wik = sh.sval[wb][2];
+            const double det = fabs(wkk * wii - wik * wik);
+            step = 0;
+            if (det >= pivot_tol) {
+              const double delta = det / fmax(aii + abswik, akk + abswik);
+              if (delta >= ustab * colmax && delta >= ustab * rowmax) {
+                step = 2;
+                kpiv = imax;
+              }
+            }
           }
+          if (step == 1 && kpiv == imax && own && r >= k) {
+            s = s2;
+            W[r + (size_t)kw * ldw] = s2;
+          }
+        } else {
+          step = 0;
         }
-        if (step == 1 && kpiv == imax && own && r >= k) {
-          s = s2;
-          W[r + (size_t)kw * ldw] = s2;
-        }
-      } else {
-        step = 0;
       }
     }
 
@@ -1390,31 +1428,32 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
       __syncthreads();
       if (k < nc - 1) interchange(k, nc - 1, kw, s, s2);
       nc -= 1;
+      wb ^= 1;
+      publish(wb);
+      __syncthreads();
       continue;
     }
     const int ksrc = k + step - 1;
     if (kpiv != ksrc) {
       __syncthreads();
       interchange(ksrc, kpiv, kw + step, s, s2);
+      plain = false;
     }
     const int q = kw;
     if (step == 1) {
-      const double d = kpiv == k ? sh.sval[0] : sh.sval[1];
+      const double d = kpiv == k ? sh.sval[wb][0] : sh.sval[wb][1];
+      const double dinv = kpiv == k ? sh.sval[wb][3] : 1.0 / d;
       if (own && r >= k) {
         double v;
         if (r == k)
           v = d;
         else
-          v = zero ? 0.0 : s * (1.0 / d);
+          v = zero ? 0.0 : s * dinv;
         Ls[q * NTH + tid] = v;
       }
-      if (tid == 0) {
-        sh.dd[q] = d;
-        sh.two[q] = 0;
-        count_inertia(1, d, 0.0, 0.0, zero, sh.cnt);
-      }
+      if (tid == 0) count_inertia(1, d, 0.0, 0.0, zero, sh.cnt);
     } else {
-      const double d11 = sh.sval[0], d21 = sh.sval[2], d22 = sh.sval[1];
+      const double d11 = sh.sval[wb][0], d21 = sh.sval[wb][2], d22 = sh.sval[wb][1];
       const double detinv = 1.0 / (d11 * d22 - d21 * d21);
       const double i00 = d22 * detinv, i01 = -d21 * detinv, i11 = d11 * detinv;
       if (own && r >= k + 2) {
@@ -1427,18 +1466,20 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
         Ls[(q + 1) * NTH + q + 1] = d22;
         vars[k] = -(vars[k] + 1);
         vars[k + 1] = -(vars[k + 1] + 1);
-        sh.dd[q] = d11;
-        sh.de[q] = d21;
-        sh.dd[q + 1] = d22;
-        sh.two[q] = 1;
-        sh.two[q + 1] = 2;
         count_inertia(2, d11, d21, d22, false, sh.cnt);
       }
     }
     k += step;
     kw += step;
-    __syncthreads();
+    wb ^= 1;
+    if (!plain || step != 1) {
+      // The next pivot row's W row was not published speculatively.
+      __syncthreads();
+      publish(wb);
+      __syncthreads();
+    }
   }
+  __syncthreads();
   panel_store<NTH>(F, ld, m, kstart, pw, Ls);
 }
 
