@@ -227,6 +227,7 @@ _SIGS.update(
         "amigo_ipm_ir_update": (_I, [_PP, _PS, _I, _V, _V, _V, _V, _V, _V, _V]),
         "amigo_assembly_last_error": (ctypes.c_char_p, []),
         "amigo_elem_num_inputs": (_I, [_I]),
+        "amigo_elem_register": (_I, [_I, _I, _V]),
         "amigo_elements_eval": (_I, [_I, _I, _V, _V, _I, _V, _D, _V, _V, _V, _V]),
         "amigo_gather_sum": (_I, [_I, _V, _V, _V, _V, _I, _V]),
         "amigo_sum_partials": (_I, [_V, _I, _V, _c_dbl_p, _V]),

@@ -365,6 +365,19 @@ __global__ void sum_partials_kernel(const double* part, int nb, double* out) {
   }
 }
 
+// Generated element types: evaluator and input count by type.
+struct GenType {
+  amigo_elem_eval_fn fn = nullptr;
+  int nc = -1;
+};
+GenType g_gen[AMIGO_ELEM_MAX_GENERATED];
+
+const GenType* gen_type(int type) {
+  const int t = type - AMIGO_ELEM_GENERATED;
+  if (t < 0 || t >= AMIGO_ELEM_MAX_GENERATED || !g_gen[t].fn) return nullptr;
+  return &g_gen[t];
+}
+
 int set_err(const char* w, hipError_t e) {
   g_asm_err = std::string(w) + ": " + hipGetErrorString(e);
   return -2;
@@ -376,7 +389,19 @@ extern "C" {
 
 const char* amigo_assembly_last_error(void) { return g_asm_err.c_str(); }
 
+int amigo_elem_register(int type, int nc, amigo_elem_eval_fn fn) {
+  const int t = type - AMIGO_ELEM_GENERATED;
+  if (t < 0 || t >= AMIGO_ELEM_MAX_GENERATED || nc <= 0 || fn == nullptr) {
+    g_asm_err = "generated element type out of range or no evaluator";
+    return -1;
+  }
+  g_gen[t].fn = fn;
+  g_gen[t].nc = nc;
+  return 0;
+}
+
 int amigo_elem_num_inputs(int type) {
+  if (const GenType* gt = gen_type(type)) return gt->nc;
   switch (type) {
     case AMIGO_ELEM_LINK:
       return 2;
@@ -415,6 +440,12 @@ int amigo_elements_eval_masked(int type, int E, const int* idx, const double* da
   if ((blk_grp == nullptr) != (grp_active == nullptr)) {
     g_asm_err = "blk_grp and grp_active go together";
     return -1;
+  }
+  if (const GenType* gt = gen_type(type)) {
+    const int rc = gt->fn(E, idx, data, ndata, x, alpha, alpha_blk, blk_grp, grp_active, gbuf,
+                          hbuf, lpart, stream);
+    if (rc != 0) g_asm_err = "generated element evaluator failed";
+    return rc;
   }
   const int nb = (E + NT - 1) / NT;
   hipStream_t s = (hipStream_t)stream;

@@ -29,6 +29,27 @@ const char* amigo_assembly_last_error(void);
 /* Number of inputs per element of a type (-1 if unknown). */
 int amigo_elem_num_inputs(int type);
 
+/* Element evaluator of a generated element type: the arguments of
+ * amigo_elements_eval_masked without the type. */
+typedef int (*amigo_elem_eval_fn)(int E, const int* idx, const double* data, int ndata,
+                                  const double* x, double alpha, const double* alpha_blk,
+                                  const int* blk_grp, const int* grp_active, double* gbuf,
+                                  double* hbuf, double* lpart, void* stream);
+
+/* First type number of generated element types; AMIGO_ELEM_MAX_GENERATED
+ * of them can be registered. */
+#define AMIGO_ELEM_GENERATED 100
+#define AMIGO_ELEM_MAX_GENERATED 256
+
+/* Register (or replace) the evaluator of generated element type `type`
+ * (AMIGO_ELEM_GENERATED <= type < AMIGO_ELEM_GENERATED +
+ * AMIGO_ELEM_MAX_GENERATED) with nc inputs per element: every evaluation
+ * entry point below then dispatches that type to fn.  The kernels come from
+ * component expressions (amigo_amd.codegen), the counterpart of the
+ * reference's generated component groups (amigo/component.py:835-1167,
+ * amigo/model.py:1409-1622). */
+int amigo_elem_register(int type, int nc, amigo_elem_eval_fn fn);
+
 /* Per-element gradient (E x NC) and Hessian (E x NC x NC, row-major) of
  * alpha * f_e + lambda^T c_e at x; idx is E x NC (-1 = absent input), data
  * is E x ndata.  gbuf / hbuf / lpart may be NULL; lpart receives one partial
