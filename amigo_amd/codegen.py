@@ -153,7 +153,7 @@ class ElementDerivatives:
         self.nc = comp.num_inputs()
         self.ndata = comp.num_data()
         L = comp.lagrangian().node
-        memo = {}
+        memo = self._memo = {}
         self.L = L
         self.g = [ex.derivative(L, i, memo) for i in range(self.nc)]
         self.H = [[None] * self.nc for _ in range(self.nc)]
@@ -164,6 +164,22 @@ class ElementDerivatives:
         self.structure = np.array(
             [[not ex._is_const(self.H[i][j], 0.0) for j in range(self.nc)] for i in range(self.nc)]
         )
+
+    def data_derivatives(self, z, d, alpha=1.0):
+        """(dL/dd, dg/dd) with numpy: E x ndata and E x nc x ndata, the
+        derivatives of the Lagrangian and of its gradient with respect to
+        the element's data values (post-optimality sensitivities)."""
+        z = np.atleast_2d(np.asarray(z, np.float64))
+        d = np.atleast_2d(np.asarray(d, np.float64))
+        memo = self._memo
+        Ld = [ex.derivative(self.L, -(j + 1), memo) for j in range(self.ndata)]
+        gd = [ex.derivative(self.g[i], -(j + 1), memo) for i in range(self.nc) for j in range(self.ndata)]
+        vals = ex.evaluate(Ld + gd, z, d, alpha=alpha)
+        E = z.shape[0]
+        dL = np.stack(vals[:self.ndata], axis=1) if self.ndata else np.zeros((E, 0))
+        dg = (np.stack(vals[self.ndata:], axis=1).reshape(E, self.nc, self.ndata) if self.ndata
+              else np.zeros((E, self.nc, 0)))
+        return dL, dg
 
     def evaluate(self, z, d, alpha=1.0):
         """(L, g, H) with numpy for inputs z (E x nc) and data d (E x ndata)."""
@@ -354,7 +370,7 @@ class GeneratedElement:
             return self.etype
         key = (os.path.basename(self.path), self.structure.tobytes())
         if key in _REGISTERED:
-            self.etype, self._lib = _REGISTERED[key]
+            self.etype, self._lib, _ = _REGISTERED[key]
         else:
             self._lib = ctypes.CDLL(self.path)
             if self._lib.amigo_gen_num_inputs() != self.nc:
@@ -364,7 +380,7 @@ class GeneratedElement:
                 raise RuntimeError("too many generated element types")
             fn = ctypes.cast(self._lib.amigo_gen_eval, ctypes.c_void_p).value
             _lib.check(_lib.lib().amigo_elem_register(etype, self.nc, ctypes.c_void_p(fn)), "asm")
-            _REGISTERED[key] = (etype, self._lib)
+            _REGISTERED[key] = (etype, self._lib, self)
             self.etype = etype
         if self.etype in P.STRUCTURE and not np.array_equal(P.STRUCTURE[self.etype], self.structure):
             raise ValueError(f"element type {self.etype} is registered with another structure")
@@ -374,6 +390,11 @@ class GeneratedElement:
 
 
 _REGISTERED = {}
+
+
+def registered():
+    """The generated element types registered in this process."""
+    return [ge for _, _, ge in _REGISTERED.values()]
 
 
 def element_type(comp, cache_dir=None, build=True, structure=None):

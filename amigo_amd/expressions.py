@@ -284,14 +284,17 @@ def _c(v):
 
 
 def derivative(node, q, memo):
-    """d node / d input q as a node (memo: dict keyed by (node id, q))."""
+    """d node / d input q as a node (memo: dict keyed by (node id, q)); a
+    negative q = -(j + 1) differentiates with respect to data value j."""
     key = (node.id, q)
     r = memo.get(key)
     if r is not None:
         return r
     op = node.op
-    if op in ("const", "data", "alpha"):
+    if op in ("const", "alpha"):
         r = _c(0.0)
+    elif op == "data":
+        r = _c(1.0 if q < 0 and node.val == -q - 1 else 0.0)
     elif op == "in":
         r = _c(1.0 if node.val == q else 0.0)
     elif op == "lt":

@@ -118,6 +118,14 @@ class Optimizer:
         """Primal and multiplier values in the unscaled space (host copy)."""
         return self._batch.get_solution(0)
 
+    def compute_post_opt_derivatives(self, of=[], wrt=[], method="adjoint"):
+        """Derivatives of solution entries (or "objective") with respect to
+        element data (group, element, slot) at the optimum
+        (post_optimization.py:23-77); see amigo_amd.post_optimization."""
+        from .post_optimization import PostOptimization
+
+        return PostOptimization(self).compute_post_opt_derivatives(of, wrt, method)
+
 
 def _write_log(iterations):
     for it in iterations:

@@ -34,7 +34,13 @@ from .optimizer import Optimizer
 
 def build_problem(spec, device="cuda"):
     """Problem of a request spec: {"type": "double_integrator" | "cart_pole" |
-    "elliptic", sizes, seed}."""
+    "elliptic", sizes, seed}, or {"problem": document} with a problem
+    document of amigo_amd.wire (a dict or its JSON text)."""
+    if "problem" in spec:
+        from . import wire
+
+        doc = spec["problem"]
+        return wire.loads(doc, device=device) if isinstance(doc, str) else wire.problem_from_dict(doc, device=device)
     kind = spec["type"]
     if kind == "double_integrator":
         from .problems import double_integrator
