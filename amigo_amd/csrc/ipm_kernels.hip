@@ -8,6 +8,8 @@
 // the results are copied to the host.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -408,37 +410,61 @@ __global__ void max_step_kernel(BT B, amigo_ipm_problem P, amigo_ipm_state S, do
   }
 }
 
-__global__ void max_step_finish(BT B, int K, amigo_ipm_problem P, const double* ws, int nb,
-                                double* out) {
+// The step bounds of each problem from its partial rows: the smallest
+// (value, index) pair in lexicographic order, which no reduction order can
+// change.  One 256-thread workgroup per problem (block k for problem k).
+__device__ __forceinline__ void step_min(double& a, double& k, double b, double l) {
+  if (b < a || (b == a && l < k)) {
+    a = b;
+    k = l;
+  }
+}
+
+__global__ __launch_bounds__(256) void max_step_finish(BT B, int K, amigo_ipm_problem P,
+                                                       const double* ws, int nb, double* out) {
+  __shared__ double sv[4][256];
+  const int t = threadIdx.x;
   if (B.prob) {
-    // One thread per problem: its partial rows and its own index list.
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.x;
     if (k >= K || (B.active && B.active[k] == 0)) return;
     nb = B.count[k];
     ws += (size_t)B.first[k] * 4;
     P.primal_indices += B.pseg[k];
     out += (size_t)k * 4;
-  } else if (threadIdx.x != 0) {
-    return;
   }
   double ax = 1.0, az = 1.0;
   double kx = 9.0e18, kz = 9.0e18;
-  for (int b = 0; b < nb; b++) {
-    const double a = ws[4 * b], k = ws[4 * b + 1];
-    if (a < ax || (a == ax && k < kx)) {
-      ax = a;
-      kx = k;
-    }
-    const double c = ws[4 * b + 2], l = ws[4 * b + 3];
-    if (c < az || (c == az && l < kz)) {
-      az = c;
-      kz = l;
-    }
+  for (int b = t; b < nb; b += 256) {
+    step_min(ax, kx, ws[4 * b], ws[4 * b + 1]);
+    step_min(az, kz, ws[4 * b + 2], ws[4 * b + 3]);
   }
-  out[0] = ax;
-  out[1] = kx < 9.0e18 ? (double)P.primal_indices[(long long)kx / 2] : -1.0;
-  out[2] = az;
-  out[3] = kz < 9.0e18 ? (double)P.primal_indices[(long long)kz / 2] : -1.0;
+  sv[0][t] = ax;
+  sv[1][t] = kx;
+  sv[2][t] = az;
+  sv[3][t] = kz;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      double a = sv[0][t], ka = sv[1][t], c = sv[2][t], kc = sv[3][t];
+      step_min(a, ka, sv[0][t + w], sv[1][t + w]);
+      step_min(c, kc, sv[2][t + w], sv[3][t + w]);
+      sv[0][t] = a;
+      sv[1][t] = ka;
+      sv[2][t] = c;
+      sv[3][t] = kc;
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    ax = sv[0][0];
+    kx = sv[1][0];
+    az = sv[2][0];
+    kz = sv[3][0];
+    out[0] = ax;
+    out[1] = kx < 9.0e18 ? (double)P.primal_indices[(long long)kx / 2] : -1.0;
+    out[2] = az;
+    out[3] = kz < 9.0e18 ? (double)P.primal_indices[(long long)kz / 2] : -1.0;
+  }
 }
 
 // Trial step (interior_point_backend.h:375-392).
@@ -1057,21 +1083,37 @@ __device__ __forceinline__ double nan_max(double a, double b) {
   return (b > a || isnan(b)) && !isnan(a) ? b : a;
 }
 
+// |x| maximum of each segment (NaN if any entry is NaN): blockIdx.x is the
+// segment, blockIdx.y strides over it, and the blocks combine with an
+// atomic maximum of the bit patterns (non-negative doubles order like their
+// bits; a NaN's bits exceed every number's), so the result does not depend
+// on the order.  out starts at zero (segment_zero_kernel).
+__global__ void segment_zero_kernel(int K, unsigned long long* out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < K) out[k] = 0ull;
+}
+
 __global__ __launch_bounds__(NT) void segment_absmax_kernel(const double* __restrict__ x,
                                                             const int* __restrict__ seg,
-                                                            double* __restrict__ out) {
-  __shared__ double red[NT / 64];
+                                                            unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long red[NT / 64];
   const int k = blockIdx.x;
-  double m = 0.0;
-  for (int i = seg[k] + threadIdx.x; i < seg[k + 1]; i += NT) m = nan_max(m, fabs(x[i]));
+  unsigned long long m = 0ull;
+  for (int i = seg[k] + blockIdx.y * NT + threadIdx.x; i < seg[k + 1]; i += gridDim.y * NT) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(x[i]));
+    m = b > m ? b : m;
+  }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_xor(m, off, 64));
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(m, off, 64);
+    m = o > m ? o : m;
+  }
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double r = red[0];
-    for (int w = 1; w < NT / 64; w++) r = nan_max(r, red[w]);
-    out[k] = r;
+    unsigned long long r = red[0];
+    for (int w = 1; w < NT / 64; w++) r = red[w] > r ? red[w] : r;
+    if (r) atomicMax(out + k, r);
   }
 }
 
@@ -1159,7 +1201,7 @@ int amigo_ipm_max_step(const amigo_ipm_problem* P, const amigo_ipm_state* S,
   hipLaunchKernelGGL(max_step_kernel, dim3(nb), dim3(NT), 0, s, BT{}, *P, *S, tau, nullptr,
                      px, dzl, dzu, ws);
   double* out = ws + (size_t)MAXB * 4;
-  hipLaunchKernelGGL(max_step_finish, dim3(1), dim3(64), 0, s, BT{}, 1, *P, ws, nb, out);
+  hipLaunchKernelGGL(max_step_finish, dim3(1), dim3(256), 0, s, BT{}, 1, *P, ws, nb, out);
   LAUNCH_CHECK("max_step_kernel");
   hipError_t e = hipMemcpyAsync(out4, out, 4 * sizeof(double), hipMemcpyDeviceToHost, s);
   if (e != hipSuccess) return set_err("hipMemcpyAsync", e);
@@ -1488,7 +1530,7 @@ int amigo_ipm_batch_max_step(const amigo_ipm_batch* B, const amigo_ipm_problem* 
   const int K = B->num_problems;
   hipLaunchKernelGGL(max_step_kernel, dim3(B->nblk_primal), dim3(NT), 0, s, bt, *P, *S, 0.0,
                      tau, px, dzl, dzu, ws);
-  hipLaunchKernelGGL(max_step_finish, dim3((K + 63) / 64), dim3(64), 0, s, bt, K, *P, ws, 0,
+  hipLaunchKernelGGL(max_step_finish, dim3(K), dim3(256), 0, s, bt, K, *P, ws, 0,
                      out_dev);
   LAUNCH_CHECK("max_step_kernel");
   if (out4) {
@@ -1641,7 +1683,13 @@ int amigo_segment_absmax(const double* x, const int* seg, int K, double* out_dev
   if (K <= 0) return 0;
   if (x == nullptr || seg == nullptr || out_dev == nullptr) return -1;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(segment_absmax_kernel, dim3(K), dim3(NT), 0, s, x, seg, out_dev);
+  unsigned long long* ob = reinterpret_cast<unsigned long long*>(out_dev);
+  hipLaunchKernelGGL(segment_zero_kernel, dim3((K + 255) / 256), dim3(256), 0, s, K, ob);
+  LAUNCH_CHECK("segment_zero_kernel");
+  // Blocks per segment: 64 for one segment, fewer as the segments fill the
+  // device themselves.
+  const int per = std::max(1, std::min(64, 4096 / K));
+  hipLaunchKernelGGL(segment_absmax_kernel, dim3(K, per), dim3(NT), 0, s, x, seg, ob);
   LAUNCH_CHECK("segment_absmax_kernel");
   if (out_host) {
     hipError_t e = hipMemcpyAsync(out_host, out_dev, sizeof(double) * (size_t)K,

@@ -1864,21 +1864,27 @@ __global__ __launch_bounds__(NT) void factor_stats_kernel(DevState S, int K) {
 // Per-block inertia of a block-diagonal matrix: block b owns nodes
 // seg[b]..seg[b+1]; one thread per block sums its nodes' pivot signs in
 // node order (integer sums: exact).
-__global__ void batch_inertia_kernel(int nb, const int* __restrict__ seg,
-                                     const int* __restrict__ ninert, int* __restrict__ out) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+// Inertia counts of each block of a batch: workgroup b sums the per-node
+// counts of block b (integer sums: any order gives the same result).
+__global__ __launch_bounds__(256) void batch_inertia_kernel(int nb, const int* __restrict__ seg,
+                                                            const int* __restrict__ ninert,
+                                                            int* __restrict__ out) {
+  __shared__ int red[4][256 / 64];
+  const int b = blockIdx.x, t = threadIdx.x;
   if (b >= nb) return;
-  int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-  for (int k = seg[b]; k < seg[b + 1]; k++) {
-    c0 += ninert[4 * k + 0];
-    c1 += ninert[4 * k + 1];
-    c2 += ninert[4 * k + 2];
-    c3 += ninert[4 * k + 3];
+  int c[4] = {0, 0, 0, 0};
+  for (int k = seg[b] + t; k < seg[b + 1]; k += 256)
+#pragma unroll
+    for (int q = 0; q < 4; q++) c[q] += ninert[4 * k + q];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    int v = c[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((t & 63) == 0) red[q][t >> 6] = v;
   }
-  out[4 * b + 0] = c0;
-  out[4 * b + 1] = c1;
-  out[4 * b + 2] = c2;
-  out[4 * b + 3] = c3;
+  __syncthreads();
+  if (t < 4) out[4 * b + t] = red[t][0] + red[t][1] + red[t][2] + red[t][3];
 }
 
 // Symmetric equilibration by Ruiz iteration: each pass replaces
@@ -2977,7 +2983,7 @@ int LdlNumeric::finish_factor(hipStream_t stream, std::string& err) {
   if (flags[0] != 0) return flags[0];
   if (nbatch_ > 0) {
     // Per-block signs; a non-finite pivot fails only its own block.
-    hipLaunchKernelGGL(batch_inertia_kernel, dim3((nbatch_ + 255) / 256), dim3(256), 0, stream,
+    hipLaunchKernelGGL(batch_inertia_kernel, dim3(nbatch_), dim3(256), 0, stream,
                        nbatch_, (const int*)d_batch_seg_, (const int*)S_.ninert, d_batch_inert_);
     HIPCHK(hipGetLastError());
     binert_.resize(4 * (size_t)nbatch_);
