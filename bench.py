@@ -69,6 +69,8 @@ def parse():
     ap.add_argument("--serve-lockstep-requests", type=int, default=96,
                     help="requests per arrival rate of the lockstep-batching server")
     ap.add_argument("--serve-lockstep-rates", default="100,1000,5000")
+    ap.add_argument("--serve-lockstep-seconds", type=float, default=0.0,
+                    help="arrivals per rate: rate x seconds requests (0: --serve-lockstep-requests each)")
     ap.add_argument("--serve-lockstep-batches", type=int, default=4,
                     help="lockstep batches the server runs at once")
     ap.add_argument("--timing-steps", type=int, default=1,
@@ -367,7 +369,10 @@ def lockstep_serving(args):
 
     from amigo_amd.serve import BatchSolveServer, SolveServer, request_mix, run_poisson
 
-    n = args.serve_lockstep_requests
+    rates = [float(r) for r in args.serve_lockstep_rates.split(",") if r]
+    per_rate = {r: (int(np.ceil(args.serve_lockstep_seconds * r)) if args.serve_lockstep_seconds > 0
+                    else args.serve_lockstep_requests) for r in rates}
+    n = max([args.serve_lockstep_requests] + list(per_rate.values()))
     specs = request_mix(n, seed=args.seed + 11)
     ns = min(16, n)
     with SolveServer(slots=1, precision="fp64") as srv:
@@ -378,16 +383,18 @@ def lockstep_serving(args):
     # reference for the batched results.
     with SolveServer(slots=1, precision="mixed") as srv:
         seq_m = [f.result() for f in [srv.submit(sp) for sp in specs[:ns]]]
-    out = {"requests_per_rate": n,
+    out = {"requests_generated": n,
            "mix": "70% cart-pole N~U{32..1024}, 30% 2-D elliptic side~U{32..192}",
            "form": "lockstep batches of the requests queued while earlier batches run (one problem type per batch), "
                    f"up to {args.serve_lockstep_batches} batches at once on their own streams, mixed precision; "
                    "each result returned when its own problem finishes",
            "sequential_fp64_sample": ns, "sequential_fp64_solves_per_s": seq_rate, "rates": []}
     same = True
-    for rate in [float(r) for r in args.serve_lockstep_rates.split(",") if r]:
+    for rate in rates:
+        nr = per_rate[rate]
+        print(f"[bench] lockstep serving: {nr} requests at {rate:.0f} req/s", file=sys.stderr, flush=True)
         with BatchSolveServer(max_batch=512, max_batches=args.serve_lockstep_batches) as srv:
-            st = run_poisson(srv, specs, rate=rate, seed=args.seed + 12)
+            st = run_poisson(srv, specs[:nr], rate=rate, seed=args.seed + 12)
         sizes = [b for b, _ in srv.batches]
         mism = [(i, specs[i]["type"], bool(st["results"][i]["batched"]),
                  float(np.max(np.abs(a["x"] - b["x"]))), a["iterations"], b["iterations"])
@@ -396,6 +403,9 @@ def lockstep_serving(args):
         dmax = max(float(np.max(np.abs(a["x"] - b["x"]))) for a, b in zip(seq, st["results"][:ns]))
         out["rates"].append({
             "arrival_rate_per_s": rate,
+            "requests": nr,
+            "mean_arrival_span_s": nr / rate,
+            "wall_s": float(st["wall_s"]),
             "value": st["solves_per_s"],
             "unit": "solves/s",
             "p50_ms": st["p50_ms"],
