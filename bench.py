@@ -181,7 +181,9 @@ def main():
         achieved = per / (ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    roof["traffic"] = _measured_traffic(args, cls)
+    if roof["traffic"] is not None:
+        roof["traffic_source"] = "profiles/elliptic2d_m256_fp64_traffic.json (copy-calibrated FETCH_SIZE + WRITE_SIZE)"
     roof["kernel"] = cls
     roof["kernel_ms_per_launch"] = ms / max(launches, 1)
     roof["launches"] = launches
@@ -227,6 +229,20 @@ def main():
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(prob, opt, nfact, nsolve, args.cpu_threads)
     print(json.dumps(result))
+
+
+def _measured_traffic(args, cls):
+    """HBM bytes per launch of the dominant class, from the copy-calibrated
+    FETCH_SIZE / WRITE_SIZE passes recorded for this workload in profiles/
+    (None for a workload or class without a recorded measurement)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "elliptic2d_m256_fp64_traffic.json")
+    if not (args.dim == 2 and args.m == 256 and args.precision == "fp64" and args.order == "ND"
+            and not args.no_scaling and os.path.exists(path)):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    return rec["bytes_per_launch"] if rec.get("kernel_class") == cls else None
 
 
 def _inertia_of_last_solve(opt, before):
