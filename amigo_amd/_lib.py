@@ -186,6 +186,12 @@ class IpmBatch(ctypes.Structure):
         ("first_primal", ctypes.c_void_p),
         ("count_primal", ctypes.c_void_p),
         ("active", ctypes.c_void_p),
+        ("nlaunch_total", ctypes.c_int),
+        ("launch_total", ctypes.c_void_p),
+        ("nlaunch_primal", ctypes.c_int),
+        ("launch_primal", ctypes.c_void_p),
+        ("nlaunch_problems", ctypes.c_int),
+        ("launch_problems", ctypes.c_void_p),
     ]
 
 
@@ -216,6 +222,7 @@ _SIGS.update(
         "amigo_csr_scale_symmetric_masked": (_I, [_I, _V, _V, _V, _V, _V, _V, _V]),
         "amigo_elements_eval_masked": (_I, [_I, _I, _V, _V, _I, _V, _D, _V, _V, _V, _V, _V, _V, _V]),
         "amigo_gather_sum_masked": (_I, [_I, _V, _V, _V, _V, _I, _V, _I, _V, _V]),
+        "amigo_gather_sum_listed": (_I, [_V, _V, _V, _V, _I, _V, _V, _I, ctypes.c_longlong, _V]),
         "amigo_csr_scale_symmetric": (_I, [_I, _V, _V, _V, _V, _V]),
         "amigo_ipm_constraint_scaling": (_I, [_I, _V, _V, _V, _D, _D, _V, _V]),
         "amigo_ipm_scale_rows": (_I, [_I, _V, _V, _V, _I, _V]),

@@ -345,6 +345,24 @@ __global__ void gather_sum_kernel(int n, const int* __restrict__ ptr,
   }
 }
 
+// The listed groups' outputs only: block b serves group list[b / chunks]
+// and strides over its outputs seg[g] .. seg[g+1] with the group's other
+// chunks; each output is the same ordered sum as in gather_sum_kernel.
+__global__ void gather_sum_listed_kernel(const int* __restrict__ ptr,
+                                         const long long* __restrict__ src,
+                                         const double* __restrict__ buf, double* __restrict__ out,
+                                         int accumulate, const long long* __restrict__ seg,
+                                         const int* __restrict__ list, int chunks) {
+  const int e = blockIdx.x / chunks, c = blockIdx.x - e * chunks;
+  const int g = list[e];
+  const long long hi = seg[g + 1];
+  for (long long i = seg[g] + (long long)c * NT + threadIdx.x; i < hi; i += (long long)chunks * NT) {
+    double s = accumulate ? out[i] : 0.0;
+    for (int p = ptr[i]; p < ptr[i + 1]; p++) s += buf[src[p]];
+    out[i] = s;
+  }
+}
+
 // Batch: problem k's sum runs over part[lidx[lptr[k] .. lptr[k+1]]] in list
 // order (its blocks of every element group, groups in order), the same
 // sequence sum_partials_kernel adds for the problem alone.
@@ -505,6 +523,27 @@ int amigo_gather_sum_masked(int n, const int* ptr, const long long* src, const d
                      buf, out, accumulate, seg, nseg, seg_active);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err("gather_sum_kernel", e);
+  return 0;
+}
+
+int amigo_gather_sum_listed(const int* ptr, const long long* src, const double* buf, double* out,
+                            int accumulate, const long long* seg, const int* list, int nlist,
+                            long long max_len, void* stream) {
+  if (nlist <= 0 || max_len <= 0) return 0;
+  if (!ptr || !src || !buf || !out || !seg || !list) {
+    g_asm_err = "gather_sum_listed: null array";
+    return -1;
+  }
+  long long chunks = (max_len + NT - 1) / NT;
+  if (chunks > 64) chunks = 64;
+  if ((long long)nlist * chunks > 0x7fffffffLL) {
+    g_asm_err = "gather_sum_listed: list too long";
+    return -1;
+  }
+  hipLaunchKernelGGL(gather_sum_listed_kernel, dim3(unsigned(nlist * chunks)), dim3(NT), 0,
+                     (hipStream_t)stream, ptr, src, buf, out, accumulate, seg, list, int(chunks));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err("gather_sum_listed_kernel", e);
   return 0;
 }
 

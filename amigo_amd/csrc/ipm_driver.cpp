@@ -19,6 +19,7 @@
 #include <limits>
 #include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/amigo_assembly.h"
@@ -233,39 +234,56 @@ struct State {
   amigo_ipm_state c() const { return amigo_ipm_state{x, zl, zu, sl, su}; }
 };
 
-// Small host-to-device uploads (masks, per-problem scalars) go through a
-// pinned arena that is recycled after every synchronization; results of
+// Small host-to-device uploads (masks, launch lists, per-problem scalars)
+// go through pinned staging chunks that are recycled only after a
+// synchronization (fetch()), so a device pointer an upload returns stays
+// valid, and its bytes unchanged, until the next fetch().  Uploading bytes
+// identical to an earlier upload since that fetch returns the earlier
+// copy: the uploads are read-only inputs of the kernels.  Results of
 // reductions go to a device arena read back in one copy.
 class Exchange {
  public:
   void init(size_t up_bytes, size_t res_doubles) {
-    up_cap_ = up_bytes;
+    chunk_bytes_ = up_bytes;
     res_cap_ = res_doubles;
-    hip_ok(hipHostMalloc((void**)&up_host_, up_cap_), "hipHostMalloc");
-    hip_ok(hipMalloc((void**)&up_dev_, up_cap_), "hipMalloc");
+    add_chunk(chunk_bytes_);
     hip_ok(hipHostMalloc((void**)&res_host_, res_cap_ * sizeof(double)), "hipHostMalloc");
     hip_ok(hipMalloc((void**)&res_dev_, res_cap_ * sizeof(double)), "hipMalloc");
   }
   ~Exchange() {
-    if (up_host_) (void)hipHostFree(up_host_);
-    if (up_dev_) (void)hipFree(up_dev_);
+    for (Chunk& c : chunks_) {
+      if (c.host) (void)hipHostFree(c.host);
+      if (c.dev) (void)hipFree(c.dev);
+    }
     if (res_host_) (void)hipHostFree(res_host_);
     if (res_dev_) (void)hipFree(res_dev_);
   }
-  void* upload(const void* src, size_t bytes, hipStream_t s) {
-    const size_t need = (bytes + 255) & ~size_t(255);
-    if (up_used_ + need > up_cap_) {
-      // Every earlier upload has been consumed once the stream is idle.
-      hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
-      up_used_ = 0;
+  Exchange() = default;
+  Exchange(const Exchange&) = delete;
+  Exchange& operator=(const Exchange&) = delete;
+
+  const void* upload(const void* src, size_t bytes, hipStream_t s) {
+    const unsigned long long h = digest(src, bytes);
+    auto range = seen_.equal_range(h);
+    for (auto it = range.first; it != range.second; ++it) {
+      const Prior& q = it->second;
+      if (q.bytes == bytes && std::memcmp(q.host, src, bytes) == 0) return q.dev;
     }
-    if (need > up_cap_) raise(-1, "upload larger than the staging arena");
-    std::memcpy(up_host_ + up_used_, src, bytes);
-    void* d = up_dev_ + up_used_;
-    hip_ok(hipMemcpyAsync(d, up_host_ + up_used_, bytes, hipMemcpyHostToDevice, s),
-           "hipMemcpyAsync");
-    up_used_ += need;
-    return d;
+    const size_t need = (bytes + 255) & ~size_t(255);
+    while (chunks_[cur_].used + need > chunks_[cur_].cap) {
+      // Later chunks are unused until the next fetch(); add one when none
+      // is left (a spare chunk too small for this upload is passed over).
+      if (cur_ + 1 == chunks_.size()) add_chunk(std::max(chunk_bytes_, need));
+      cur_++;
+    }
+    Chunk& c = chunks_[cur_];
+    char* hp = c.host + c.used;
+    char* dp = c.dev + c.used;
+    std::memcpy(hp, src, bytes);
+    hip_ok(hipMemcpyAsync(dp, hp, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    c.used += need;
+    seen_.emplace(h, Prior{bytes, hp, dp});
+    return dp;
   }
   // A result region of count doubles; index into host() after fetch().
   size_t slot(size_t count) {
@@ -281,16 +299,54 @@ class Exchange {
                             hipMemcpyDeviceToHost, s),
              "hipMemcpyAsync");
     hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
-    up_used_ = 0;
+    for (Chunk& c : chunks_) c.used = 0;
+    cur_ = 0;
+    seen_.clear();
+    generation_++;
     fetched_ = res_used_;
     res_used_ = 0;
   }
   double host(size_t off) const { return res_host_[off]; }
+  // Changes at every fetch(): device pointers of earlier uploads are void.
+  unsigned long long generation() const { return generation_; }
 
  private:
-  char* up_host_ = nullptr;
-  char* up_dev_ = nullptr;
-  size_t up_cap_ = 0, up_used_ = 0;
+  struct Chunk {
+    char* host = nullptr;
+    char* dev = nullptr;
+    size_t cap = 0, used = 0;
+  };
+  struct Prior {
+    size_t bytes;
+    const char* host;
+    const char* dev;
+  };
+  static unsigned long long digest(const void* src, size_t bytes) {
+    // FNV-1a over 8-byte words, then the tail bytes.
+    const unsigned char* p = static_cast<const unsigned char*>(src);
+    unsigned long long h = 1469598103934665603ull ^ bytes;
+    size_t i = 0;
+    for (; i + 8 <= bytes; i += 8) {
+      unsigned long long w;
+      std::memcpy(&w, p + i, 8);
+      h = (h ^ w) * 1099511628211ull;
+    }
+    for (; i < bytes; i++) h = (h ^ p[i]) * 1099511628211ull;
+    return h;
+  }
+  void alloc_chunk(Chunk& c, size_t bytes) {
+    c.cap = bytes;
+    hip_ok(hipHostMalloc((void**)&c.host, bytes), "hipHostMalloc");
+    hip_ok(hipMalloc((void**)&c.dev, bytes), "hipMalloc");
+  }
+  void add_chunk(size_t bytes) {
+    chunks_.emplace_back();
+    alloc_chunk(chunks_.back(), bytes);
+  }
+  std::vector<Chunk> chunks_;
+  size_t cur_ = 0, chunk_bytes_ = 0;
+  std::unordered_multimap<unsigned long long, Prior> seen_;
+  unsigned long long generation_ = 0;
   double* res_host_ = nullptr;
   double* res_dev_ = nullptr;
   size_t res_cap_ = 0, res_used_ = 0, fetched_ = 0;
@@ -377,7 +433,7 @@ struct amigo_ipm_driver {
   Exchange ex;
   amigo_ipm_options o{};
   std::vector<Prob> P;
-  bool mask_all_ldl = true;
+  Set ldl_set;  // the set the factorization's mask holds (empty: not known)
   bool scaling = false;
   double phase_s[PH_COUNT] = {0, 0, 0, 0, 0, 0, 0};
   double host_s = 0.0;
@@ -444,14 +500,103 @@ struct amigo_ipm_driver {
       P[k].n_primal = pseg[k + 1] - pseg[k];
       P[k].n_dual = cseg[k + 1] - cseg[k];
     }
+    // Host copies of the block tables and the Hessian segments (the launch
+    // lists of small sets are built from them).
+    hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    auto down = [&](auto& h, const auto* dv, size_t count) {
+      h.resize(count);
+      hip_ok(hipMemcpy(h.data(), dv, sizeof(h[0]) * count, hipMemcpyDeviceToHost), "hipMemcpy");
+    };
+    down(first_t, d.first_total, K);
+    down(count_t, d.count_total, K);
+    down(first_p, d.first_primal, K);
+    down(count_p, d.count_primal, K);
+    down(hseg, d.hseg64, (size_t)K + 1);
   }
 
   // ---- sets, masks, scalars ------------------------------------------------
-  const int* mask(const Set& m) {
-    if (all(m)) return nullptr;
+  // A set of problems as the kernels take it: a mask (null = every problem)
+  // and, when the set is at most a quarter of the batch, lists of its
+  // problems and of their blocks, so that launches cover only their work
+  // (with the results of the masked full launches).  Built once per set
+  // between two fetches.
+  struct Sel {
+    const int* mask = nullptr;
+    const int* probs = nullptr;
+    int nprobs = 0;
+    const int* blk_t = nullptr;
+    int nblk_t = 0;
+    const int* blk_p = nullptr;
+    int nblk_p = 0;
+    int max_x = 0, max_c = 0;
+    long long max_h = 0;
+  };
+  struct SelEntry {
+    unsigned long long gen;
+    Set set;
+    Sel sel;
+  };
+  static constexpr int kListFraction = 4;
+  std::vector<SelEntry> sel_cache;
+  std::vector<int> first_t, count_t, first_p, count_p;
+  std::vector<long long> hseg;
+
+  Sel select(const Set& A) {
+    if (all(A)) return Sel{};
+    const unsigned long long g = ex.generation();
+    for (const SelEntry& e : sel_cache)
+      if (e.gen == g && e.set == A) return e.sel;
     std::vector<int> h(K);
-    for (int k = 0; k < K; k++) h[k] = m[k] ? 1 : 0;
-    return (const int*)ex.upload(h.data(), sizeof(int) * K, s);
+    int na = 0;
+    for (int k = 0; k < K; k++) {
+      h[k] = A[k] ? 1 : 0;
+      na += h[k];
+    }
+    Sel r;
+    if (na > 0 && na * kListFraction <= K) {
+      // one upload: mask, problems, total-grid blocks, primal-grid blocks
+      for (int k = 0; k < K; k++)
+        if (A[k]) {
+          h.push_back(k);
+          r.max_x = std::max(r.max_x, xseg[k + 1] - xseg[k]);
+          r.max_c = std::max(r.max_c, cseg[k + 1] - cseg[k]);
+          r.max_h = std::max(r.max_h, hseg[k + 1] - hseg[k]);
+        }
+      for (int k = 0; k < K; k++)
+        if (A[k])
+          for (int b = first_t[k]; b < first_t[k] + count_t[k]; b++) h.push_back(b);
+      const size_t nbt = h.size() - K - na;
+      for (int k = 0; k < K; k++)
+        if (A[k])
+          for (int b = first_p[k]; b < first_p[k] + count_p[k]; b++) h.push_back(b);
+      const size_t nbp = h.size() - K - na - nbt;
+      const int* dv = (const int*)ex.upload(h.data(), sizeof(int) * h.size(), s);
+      r.mask = dv;
+      r.probs = dv + K;
+      r.nprobs = na;
+      r.blk_t = r.probs + na;
+      r.nblk_t = int(nbt);
+      r.blk_p = r.blk_t + nbt;
+      r.nblk_p = int(nbp);
+    } else {
+      r.mask = (const int*)ex.upload(h.data(), sizeof(int) * K, s);
+    }
+    sel_cache.erase(std::remove_if(sel_cache.begin(), sel_cache.end(),
+                                   [&](const SelEntry& e) { return e.gen != g; }),
+                    sel_cache.end());
+    if (sel_cache.size() >= 8) sel_cache.erase(sel_cache.begin());
+    sel_cache.push_back(SelEntry{g, A, r});
+    return r;
+  }
+  const int* mask(const Set& m) { return select(m).mask; }
+  // Rows of the listed problems (x-length and constraint segments).
+  amigo_amd::drv::Rows rows_x(const Sel& q) const {
+    return q.probs ? amigo_amd::drv::Rows{q.probs, q.nprobs, d.xseg, q.max_x}
+                   : amigo_amd::drv::Rows{};
+  }
+  amigo_amd::drv::Rows rows_c(const Sel& q) const {
+    return q.probs ? amigo_amd::drv::Rows{q.probs, q.nprobs, d.cseg, q.max_c}
+                   : amigo_amd::drv::Rows{};
   }
   const double* scalars(const std::vector<double>& v) {
     return (const double*)ex.upload(v.data(), sizeof(double) * K, s);
@@ -476,8 +621,23 @@ struct amigo_ipm_driver {
     B.first_primal = d.first_primal;
     B.count_primal = d.count_primal;
     B.active = active;
+    B.nlaunch_total = B.nlaunch_primal = B.nlaunch_problems = 0;
+    B.launch_total = B.launch_primal = B.launch_problems = nullptr;
     return B;
   }
+  amigo_ipm_batch batch(const Sel& q) const {
+    amigo_ipm_batch B = batch(q.mask);
+    if (q.probs) {
+      B.nlaunch_total = q.nblk_t;
+      B.launch_total = q.blk_t;
+      B.nlaunch_primal = q.nblk_p;
+      B.launch_primal = q.blk_p;
+      B.nlaunch_problems = q.nprobs;
+      B.launch_problems = q.probs;
+    }
+    return B;
+  }
+  amigo_ipm_batch batch(const Set& A) { return batch(select(A)); }
   amigo_ipm_problem info() const {
     return amigo_ipm_problem{d.n_primal, d.n_constraints, d.pidx, d.cidx, d.lbx, d.ubx, d.lbh};
   }
@@ -509,14 +669,18 @@ struct amigo_ipm_driver {
   // constraint rows scaled (newton_direction.py:17-27).
   void gradient(const Set& A, const double* x, double* g) {
     eval(A, x, -1.0, true, false, false);
-    const int* m = mask(A);
-    if (!m)
+    const Sel q = select(A);
+    if (!q.mask)
       asm_ok(amigo_gather_sum(d.n, d.g_ptr, d.g_src, d.gbuf, g, 0, s), "gradient gather");
+    else if (q.probs)
+      asm_ok(amigo_gather_sum_listed(d.g_ptr, d.g_src, d.gbuf, g, 0, d.xseg64, q.probs, q.nprobs,
+                                     q.max_x, s),
+             "gradient gather");
     else
-      asm_ok(amigo_gather_sum_masked(d.n, d.g_ptr, d.g_src, d.gbuf, g, 0, d.xseg64, K, m, s),
+      asm_ok(amigo_gather_sum_masked(d.n, d.g_ptr, d.g_src, d.gbuf, g, 0, d.xseg64, K, q.mask, s),
              "gradient gather");
     if (scaling) {
-      const amigo_ipm_batch B = batch(m);
+      const amigo_ipm_batch B = batch(q);
       ipm_ok(amigo_ipm_batch_scale_rows(&B, d.cidx, dc, g, 0, s), "gradient scaling");
     }
   }
@@ -524,9 +688,14 @@ struct amigo_ipm_driver {
   // (optimizer.h:573-584) once the NLP scaling is on.
   void hessian(const Set& A, const double* x, double alpha, bool scale) {
     eval(A, x, alpha, false, true, false);
-    const int* m = mask(A);
+    const Sel q = select(A);
+    const int* m = q.mask;
     if (!m)
       asm_ok(amigo_gather_sum(d.nnz, d.h_ptr, d.h_src, d.hbuf, d.hess, 0, s), "Hessian gather");
+    else if (q.probs)
+      asm_ok(amigo_gather_sum_listed(d.h_ptr, d.h_src, d.hbuf, d.hess, 0, d.hseg64, q.probs,
+                                     q.nprobs, q.max_h, s),
+             "Hessian gather");
     else
       asm_ok(amigo_gather_sum_masked(d.nnz, d.h_ptr, d.h_src, d.hbuf, d.hess, 0, d.hseg64, K, m,
                                      s),
@@ -543,9 +712,10 @@ struct amigo_ipm_driver {
   // alpha f(x) of each problem (the Lagrangian with the multipliers
   // zeroed, filter_line_search.py:19-46).
   Red objective(const Set& A, const double* x) {
-    const int* m = mask(A);
-    hip_ok(amigo_amd::drv::copy_rows(d.n, d.row_prob, m, x, xobj.p, s), "copy");
-    hip_ok(amigo_amd::drv::zero_rows(d.n_constraints, d.cidx_global, xobj.p, s), "zero rows");
+    const Sel q = select(A);
+    hip_ok(amigo_amd::drv::copy_rows(d.n, d.row_prob, q.mask, x, xobj.p, s, rows_x(q)), "copy");
+    hip_ok(amigo_amd::drv::zero_rows(d.n_constraints, d.cidx_global, xobj.p, s, rows_c(q)),
+           "zero rows");
     eval(A, xobj.p, -1.0, false, false, true);
     Red r = red_slot(1);
     asm_ok(amigo_sum_partials_batch(d.lpart, K, d.lptr, d.lidx, ex.dev(r.off), nullptr, s),
@@ -556,7 +726,7 @@ struct amigo_ipm_driver {
   // ---- interior-point kernels ----------------------------------------------
   Red reduce(int kind, const Set& A, const State& v, const double* mu, const double* a,
              const double* b = nullptr, const double* c = nullptr) {
-    const amigo_ipm_batch B = batch(mask(A));
+    const amigo_ipm_batch B = batch(A);
     const amigo_ipm_problem I = info();
     const amigo_ipm_state S = v.c();
     Red r = red_slot(4);
@@ -570,7 +740,7 @@ struct amigo_ipm_driver {
   }
   // condensed residual (backend.h:202-266); out[0] = ||res||^2
   Red residual(const Set& A, const State& v, const double* g, double* out) {
-    const amigo_ipm_batch B = batch(mask(A));
+    const amigo_ipm_batch B = batch(A);
     const amigo_ipm_problem I = info();
     const amigo_ipm_state S = v.c();
     Red r = red_slot(3);
@@ -580,16 +750,16 @@ struct amigo_ipm_driver {
     return r;
   }
   void sigma(const Set& A, const State& v, double* out) {
-    const amigo_ipm_batch B = batch(mask(A));
+    const amigo_ipm_batch B = batch(A);
     const amigo_ipm_problem I = info();
     const amigo_ipm_state S = v.c();
     ipm_ok(amigo_ipm_batch_diagonal(&B, &I, &S, out, s), "diagonal");
   }
   // update = (px, dzl(px), dzu(px)) (backend.h:302-321)
   void direction(const Set& A, const double* p) {
-    const int* m = mask(A);
-    hip_ok(amigo_amd::drv::copy_rows(d.n, d.row_prob, m, p, upd.x, s), "copy");
-    const amigo_ipm_batch B = batch(m);
+    const Sel q = select(A);
+    hip_ok(amigo_amd::drv::copy_rows(d.n, d.row_prob, q.mask, p, upd.x, s, rows_x(q)), "copy");
+    const amigo_ipm_batch B = batch(q);
     const amigo_ipm_problem I = info();
     const amigo_ipm_state S = vars.c();
     ipm_ok(amigo_ipm_batch_bound_dual_step(&B, &I, &S, scalars(per(&Prob::mu)), p, upd.zl, upd.zu,
@@ -597,7 +767,7 @@ struct amigo_ipm_driver {
            "bound dual step");
   }
   Red max_step(const Set& A, const std::vector<double>& tau) {
-    const amigo_ipm_batch B = batch(mask(A));
+    const amigo_ipm_batch B = batch(A);
     const amigo_ipm_problem I = info();
     const amigo_ipm_state S = vars.c();
     Red r = red_slot(4);
@@ -608,7 +778,7 @@ struct amigo_ipm_driver {
   }
   // temp = vars + (ax dx, az dz) (backend.h:375-392)
   void trial(const Set& A, const std::vector<double>& ax, const std::vector<double>& az) {
-    const amigo_ipm_batch B = batch(mask(A));
+    const amigo_ipm_batch B = batch(A);
     const amigo_ipm_problem I = info();
     const amigo_ipm_state S = vars.c(), T = temp.c();
     ipm_ok(amigo_ipm_batch_apply_step(&B, &I, &S, scalars(ax), scalars(az), upd.x, upd.zl, upd.zu,
@@ -616,26 +786,28 @@ struct amigo_ipm_driver {
            "apply step");
   }
   void copy_state(const Set& A, const State& dst, const State& src, int what) {
-    const amigo_ipm_batch B = batch(mask(A));
+    const amigo_ipm_batch B = batch(A);
     const amigo_ipm_state D = dst.c(), S = src.c();
     ipm_ok(amigo_ipm_batch_copy_state(&B, &D, &S, what, s), "copy state");
   }
   void copy_x(const Set& A, double* dst, const double* src) {
-    hip_ok(amigo_amd::drv::copy_rows(d.n, d.row_prob, mask(A), src, dst, s), "copy");
+    const Sel q = select(A);
+    hip_ok(amigo_amd::drv::copy_rows(d.n, d.row_prob, q.mask, src, dst, s, rows_x(q)), "copy");
   }
 
   // ---- factor and solves ---------------------------------------------------
+  // The factorization's problem mask (it builds launch lists and
+  // synchronizes, so it is set only when the set changes).
   void ldl_mask(const Set& A) {
+    if (A == ldl_set) return;
     if (all(A)) {
-      if (mask_all_ldl) return;
       ldl_ok(amigo_ldl_set_batch_mask(d.ldl, nullptr, s), "set_batch_mask");
-      mask_all_ldl = true;
-      return;
+    } else {
+      std::vector<int> h(K);
+      for (int k = 0; k < K; k++) h[k] = A[k] ? 1 : 0;
+      ldl_ok(amigo_ldl_set_batch_mask(d.ldl, h.data(), s), "set_batch_mask");
     }
-    std::vector<int> h(K);
-    for (int k = 0; k < K; k++) h[k] = A[k] ? 1 : 0;
-    ldl_ok(amigo_ldl_set_batch_mask(d.ldl, h.data(), s), "set_batch_mask");
-    mask_all_ldl = false;
+    ldl_set = A;
   }
   // Factor the shifted KKT of the problems of A (diag is the shift);
   // per-problem inertia into pos/neg/nan.
@@ -902,9 +1074,12 @@ void amigo_ipm_driver::initialize() {
 // regularization, refactored until the inertia is (n_primal, n_dual, 0).
 std::vector<char> amigo_ipm_driver::factorize(const Set& A) {
   hessian(A, vars.x, -1.0, true);
-  hip_ok(amigo_amd::drv::regularized_base(d.n, d.is_mult, d.row_prob, mask(A), dbase.p, 1e-12,
-                                          reg.p, s),
-         "regularized diagonal");
+  {
+    const Sel q = select(A);
+    hip_ok(amigo_amd::drv::regularized_base(d.n, d.is_mult, d.row_prob, q.mask, dbase.p, 1e-12,
+                                            reg.p, s, rows_x(q)),
+           "regularized diagonal");
+  }
   std::vector<char> ok(K, 0);
   Set pending = A;
   for (int k = 0; k < K; k++)
@@ -917,8 +1092,9 @@ std::vector<char> amigo_ipm_driver::factorize(const Set& A) {
       dw[k] = P[k].reg.dw();
       dcv[k] = P[k].reg.dc();
     }
-    hip_ok(amigo_amd::drv::diagonal_shift(d.n, d.is_mult, d.row_prob, mask(pending), reg.p,
-                                          scalars(dw), scalars(dcv), diag.p, s),
+    const Sel qp = select(pending);
+    hip_ok(amigo_amd::drv::diagonal_shift(d.n, d.is_mult, d.row_prob, qp.mask, reg.p, scalars(dw),
+                                          scalars(dcv), diag.p, s, rows_x(qp)),
            "diagonal shift");
     if (!captured && (d.capture_hess || d.capture_shift)) {
       if (d.capture_hess)
@@ -981,7 +1157,7 @@ std::vector<double> amigo_ipm_driver::refine(const Set& B) {
   const amigo_ipm_state S = vars.c();
   Red rn = red_slot(1);
   {
-    const amigo_ipm_batch BB = batch(mask(B));
+    const amigo_ipm_batch BB = batch(B);
     ipm_ok(amigo_ipm_batch_ir_init(&BB, &I, &S, mu, grad.p, rhs.p, px.p, dzl.p, dzu.p, ws.p,
                                    ex.dev(rn.off), nullptr, s),
            "refinement init");
@@ -991,11 +1167,11 @@ std::vector<double> amigo_ipm_driver::refine(const Set& B) {
   for (int k = 0; k < K; k++) nrm_rhs[k] = at(rn, k);
   Set R = B;
   for (int step = 0; step < 10 && any(R); step++) {
-    const int* m = mask(R);
+    const Sel q = select(R);
     hip_ok(amigo_amd::drv::kkt_matvec(d.n, d.rowp, d.cols, d.hess, diag.p, px.p, kpx.p, d.row_prob,
-                                      m, s),
+                                      q.mask, s, rows_x(q)),
            "KKT product");
-    const amigo_ipm_batch BB = batch(m);
+    const amigo_ipm_batch BB = batch(q);
     Red rr = red_slot(2);
     ipm_ok(amigo_ipm_batch_ir_residual(&BB, &I, &S, scalars(per(&Prob::mu)), grad.p, rhs.p, kpx.p,
                                        px.p, dzl.p, dzu.p, res.p, ezl.p, ezu.p, ws.p,
@@ -1017,7 +1193,7 @@ std::vector<double> amigo_ipm_driver::refine(const Set& B) {
     }
     if (!any(R)) break;
     solve(R, res.p, corr.p);
-    const amigo_ipm_batch BU = batch(mask(R));
+    const amigo_ipm_batch BU = batch(R);
     ipm_ok(amigo_ipm_batch_ir_update(&BU, &I, &S, corr.p, ezl.p, ezu.p, px.p, dzl.p, dzu.p, s),
            "refinement update");
     for (int k = 0; k < K; k++)
@@ -1260,8 +1436,9 @@ void amigo_ipm_driver::line_search(const Set& S, std::vector<Search>& ls,
       residual(soc_set, temp, grad.p, res.p);
       std::vector<double> asoc(K, 0.0);
       for (int k = 0; k < K; k++) asoc[k] = ls[k].alpha_soc;
-      hip_ok(amigo_amd::drv::soc_rhs(d.n_constraints, d.cidx_global, d.row_prob, mask(soc_set),
-                                     scalars(asoc), res.p, csoc.p, s),
+      const Sel q = select(soc_set);
+      hip_ok(amigo_amd::drv::soc_rhs(d.n_constraints, d.cidx_global, d.row_prob, q.mask,
+                                     scalars(asoc), res.p, csoc.p, s, rows_c(q)),
              "correction rhs");
       copy_x(soc_set, res.p, csoc.p);
       gradient(soc_set, vars.x, grad.p);
@@ -1648,7 +1825,7 @@ void amigo_ipm_driver::finish_step(const Set& done, const Set& accepted) {
       if (sm == 0.0) sm = std::numeric_limits<double>::min();
       smin[k] = sm;
     }
-    const amigo_ipm_batch BG = batch(mask(G));
+    const amigo_ipm_batch BG = batch(G);
     const amigo_ipm_problem I = info();
     const amigo_ipm_state S = vars.c();
     Red fl = red_slot(1);
@@ -1693,6 +1870,7 @@ void amigo_ipm_driver::finish_step(const Set& done, const Set& accepted) {
 void amigo_ipm_driver::run() {
   t0 = std::chrono::steady_clock::now();
   auto tp = t0;
+  ldl_set.clear();
   reset();
   initialize();
   toc(PH_INIT, tp);

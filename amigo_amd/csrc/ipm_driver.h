@@ -10,25 +10,42 @@ namespace amigo_amd {
 namespace drv {
 
 // ---- element-wise helper kernels (driver_kernels.hip) ----------------------
+// The rows a launch covers: all n rows (list == nullptr), or only the
+// segments seg[list[e]] .. seg[list[e] + 1] of the nlist listed problems
+// (device arrays; max_len bounds the listed segments' lengths).  Either way
+// a kernel also skips rows of problems whose active entry is zero, and
+// computes each row exactly as the full launch does.
+struct Rows {
+  const int* list = nullptr;
+  int nlist = 0;
+  const int* seg = nullptr;
+  int max_len = 0;
+};
+
 hipError_t fill(long long n, double v, double* x, hipStream_t s);
 hipError_t regularized_base(int n, const int* is_mult, const int* row_prob, const int* active,
-                            const double* base, double eps, double* reg, hipStream_t s);
+                            const double* base, double eps, double* reg, hipStream_t s,
+                            const Rows& rows = Rows{});
 hipError_t diagonal_shift(int n, const int* is_mult, const int* row_prob, const int* active,
                           const double* reg, const double* dw, const double* dc, double* shift,
-                          hipStream_t s);
+                          hipStream_t s, const Rows& rows = Rows{});
 hipError_t unit_primal(int n, const int* is_mult, double* d, hipStream_t s);
 hipError_t negate(int n, double* x, hipStream_t s);
 hipError_t kkt_matvec(int n, const int* rowp, const int* cols, const double* vals,
                       const double* shift, const double* x, double* y, const int* row_prob,
-                      const int* active, hipStream_t s);
+                      const int* active, hipStream_t s, const Rows& rows = Rows{});
 hipError_t gather(int n, const long long* idx, const double* src, double* dst, hipStream_t s);
+// The constraint-list kernels take rows over constraint numbers j (cidx[j]).
 hipError_t set_multipliers(int nc, const long long* cidx, const int* row_prob, const int* keep,
-                           const int* active, const double* src, double* x, hipStream_t s);
-hipError_t zero_rows(int nc, const long long* cidx, double* x, hipStream_t s);
+                           const int* active, const double* src, double* x, hipStream_t s,
+                           const Rows& rows = Rows{});
+hipError_t zero_rows(int nc, const long long* cidx, double* x, hipStream_t s,
+                     const Rows& rows = Rows{});
 hipError_t soc_rhs(int nc, const long long* cidx, const int* row_prob, const int* active,
-                   const double* alpha, const double* res, double* c, hipStream_t s);
+                   const double* alpha, const double* res, double* c, hipStream_t s,
+                   const Rows& rows = Rows{});
 hipError_t copy_rows(int n, const int* row_prob, const int* active, const double* src, double* dst,
-                     hipStream_t s);
+                     hipStream_t s, const Rows& rows = Rows{});
 hipError_t constraint_scale_vectors(int n, int nc, const long long* cidx, const double* dc,
                                     const double* lbh0, double* sv, double* lbh, hipStream_t s);
 

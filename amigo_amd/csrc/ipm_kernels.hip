@@ -50,7 +50,9 @@ struct BT {
   const int* pseg;
   const int* cseg;
   const int* xseg;
-  const int* active;  // [K] nonzero = the problem takes part (null = all)
+  const int* active;    // [K] nonzero = the problem takes part (null = all)
+  const int* launch;    // block numbers of a compacted launch (null = block b is blockIdx.x)
+  const int* problems;  // problem numbers of a compacted final pass (null = blockIdx.x)
 };
 
 struct View {
@@ -70,11 +72,12 @@ __device__ __forceinline__ View view(const BT& B, amigo_ipm_problem& P, amigo_ip
     v.skip = false;
     return v;
   }
-  const int k = B.prob[blockIdx.x];
+  const int b = B.launch ? B.launch[blockIdx.x] : (int)blockIdx.x;
+  const int k = B.prob[b];
   v.k = k;
-  v.bx = blockIdx.x - B.first[k];
+  v.bx = b - B.first[k];
   v.gx = B.count[k];
-  v.slot = blockIdx.x;
+  v.slot = b;
   v.skip = B.active && B.active[k] == 0;
   v.p0 = B.pseg[k];
   v.c0 = B.cseg[k];
@@ -166,7 +169,7 @@ template <int NV>
 __global__ void finish_batch_kernel(BT B, const double* ws, int o0, int o1, int o2, int o3,
                                     double* out) {
   __shared__ double red[(NT / 64) * 4];
-  const int k = blockIdx.x;
+  const int k = B.problems ? B.problems[blockIdx.x] : (int)blockIdx.x;
   if (B.active && B.active[k] == 0) return;
   const int ops[4] = {o0, o1, o2, o3};
   const int nb = B.count[k];
@@ -425,7 +428,7 @@ __global__ __launch_bounds__(256) void max_step_finish(BT B, int K, amigo_ipm_pr
   __shared__ double sv[4][256];
   const int t = threadIdx.x;
   if (B.prob) {
-    const int k = blockIdx.x;
+    const int k = B.problems ? B.problems[blockIdx.x] : (int)blockIdx.x;
     if (k >= K || (B.active && B.active[k] == 0)) return;
     nb = B.count[k];
     ws += (size_t)B.first[k] * 4;
@@ -1436,17 +1439,32 @@ int amigo_ipm_ir_update(const amigo_ipm_problem* P, const amigo_ipm_state* S, in
 // ---- batches -----------------------------------------------------------------
 
 static BT bt_total(const amigo_ipm_batch* B) {
-  return BT{B->blk_total, B->first_total, B->count_total, B->pseg, B->cseg, B->xseg, B->active};
+  return BT{B->blk_total, B->first_total, B->count_total, B->pseg,          B->cseg,
+            B->xseg,      B->active,      B->launch_total, B->launch_problems};
 }
 static BT bt_primal(const amigo_ipm_batch* B) {
-  return BT{B->blk_primal, B->first_primal, B->count_primal, B->pseg, B->cseg, B->xseg,
-            B->active};
+  return BT{B->blk_primal, B->first_primal, B->count_primal,  B->pseg,          B->cseg,
+            B->xseg,       B->active,       B->launch_primal, B->launch_problems};
+}
+// Grids of a batched launch: the listed blocks / problems, or all of them.
+static dim3 grid_total(const amigo_ipm_batch* B) {
+  return dim3(B->launch_total ? B->nlaunch_total : B->nblk_total);
+}
+static dim3 grid_primal(const amigo_ipm_batch* B) {
+  return dim3(B->launch_primal ? B->nlaunch_primal : B->nblk_primal);
+}
+static dim3 grid_problems(const amigo_ipm_batch* B) {
+  return dim3(B->launch_problems ? B->nlaunch_problems : B->num_problems);
 }
 
 static int batch_ok(const amigo_ipm_batch* B) {
   if (!B || B->num_problems < 1 || B->nblk_total < 1 || B->nblk_primal < 1 || !B->pseg ||
       !B->cseg || !B->xseg || !B->blk_total || !B->first_total || !B->count_total ||
-      !B->blk_primal || !B->first_primal || !B->count_primal) {
+      !B->blk_primal || !B->first_primal || !B->count_primal ||
+      (B->launch_total && (B->nlaunch_total < 1 || B->nlaunch_total > B->nblk_total)) ||
+      (B->launch_primal && (B->nlaunch_primal < 1 || B->nlaunch_primal > B->nblk_primal)) ||
+      (B->launch_problems &&
+       (B->nlaunch_problems < 1 || B->nlaunch_problems > B->num_problems))) {
     g_ipm_err = "invalid batch description";
     return -1;
   }
@@ -1459,19 +1477,19 @@ static int batch_finish(const amigo_ipm_batch* B, BT bt, int NV, const int* ops,
   const int K = B->num_problems;
   switch (NV) {
     case 1:
-      hipLaunchKernelGGL(finish_batch_kernel<1>, dim3(K), dim3(NT), 0, s, bt, ws, ops[0], 0, 0,
+      hipLaunchKernelGGL(finish_batch_kernel<1>, grid_problems(B), dim3(NT), 0, s, bt, ws, ops[0], 0, 0,
                          0, out_dev);
       break;
     case 2:
-      hipLaunchKernelGGL(finish_batch_kernel<2>, dim3(K), dim3(NT), 0, s, bt, ws, ops[0], ops[1],
+      hipLaunchKernelGGL(finish_batch_kernel<2>, grid_problems(B), dim3(NT), 0, s, bt, ws, ops[0], ops[1],
                          0, 0, out_dev);
       break;
     case 3:
-      hipLaunchKernelGGL(finish_batch_kernel<3>, dim3(K), dim3(NT), 0, s, bt, ws, ops[0], ops[1],
+      hipLaunchKernelGGL(finish_batch_kernel<3>, grid_problems(B), dim3(NT), 0, s, bt, ws, ops[0], ops[1],
                          ops[2], 0, out_dev);
       break;
     default:
-      hipLaunchKernelGGL(finish_batch_kernel<4>, dim3(K), dim3(NT), 0, s, bt, ws, ops[0], ops[1],
+      hipLaunchKernelGGL(finish_batch_kernel<4>, grid_problems(B), dim3(NT), 0, s, bt, ws, ops[0], ops[1],
                          ops[2], ops[3], out_dev);
       break;
   }
@@ -1494,7 +1512,7 @@ int amigo_ipm_batch_residual(const amigo_ipm_batch* B, const amigo_ipm_problem* 
   if (batch_ok(B)) return -1;
   hipStream_t s = (hipStream_t)stream;
   const BT bt = bt_total(B);
-  hipLaunchKernelGGL(residual_kernel, dim3(B->nblk_total), dim3(NT), 0, s, bt, *P, *S, 0.0, mu,
+  hipLaunchKernelGGL(residual_kernel, grid_total(B), dim3(NT), 0, s, bt, *P, *S, 0.0, mu,
                      grad, res, ws);
   LAUNCH_CHECK("residual_kernel");
   const int ops[3] = {SUM, SUM, SUM};
@@ -1504,7 +1522,7 @@ int amigo_ipm_batch_residual(const amigo_ipm_batch* B, const amigo_ipm_problem* 
 int amigo_ipm_batch_diagonal(const amigo_ipm_batch* B, const amigo_ipm_problem* P,
                              const amigo_ipm_state* S, double* diag, void* stream) {
   if (batch_ok(B)) return -1;
-  hipLaunchKernelGGL(diagonal_kernel, dim3(B->nblk_total), dim3(NT), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(diagonal_kernel, grid_total(B), dim3(NT), 0, (hipStream_t)stream,
                      bt_total(B), *P, *S, diag);
   LAUNCH_CHECK("diagonal_kernel");
   return 0;
@@ -1514,7 +1532,7 @@ int amigo_ipm_batch_bound_dual_step(const amigo_ipm_batch* B, const amigo_ipm_pr
                                     const amigo_ipm_state* S, const double* mu,
                                     const double* px, double* dzl, double* dzu, void* stream) {
   if (batch_ok(B)) return -1;
-  hipLaunchKernelGGL(bound_dual_step_kernel, dim3(B->nblk_primal), dim3(NT), 0,
+  hipLaunchKernelGGL(bound_dual_step_kernel, grid_primal(B), dim3(NT), 0,
                      (hipStream_t)stream, bt_primal(B), *P, *S, 0.0, mu, px, dzl, dzu);
   LAUNCH_CHECK("bound_dual_step_kernel");
   return 0;
@@ -1528,9 +1546,9 @@ int amigo_ipm_batch_max_step(const amigo_ipm_batch* B, const amigo_ipm_problem* 
   hipStream_t s = (hipStream_t)stream;
   const BT bt = bt_primal(B);
   const int K = B->num_problems;
-  hipLaunchKernelGGL(max_step_kernel, dim3(B->nblk_primal), dim3(NT), 0, s, bt, *P, *S, 0.0,
+  hipLaunchKernelGGL(max_step_kernel, grid_primal(B), dim3(NT), 0, s, bt, *P, *S, 0.0,
                      tau, px, dzl, dzu, ws);
-  hipLaunchKernelGGL(max_step_finish, dim3(K), dim3(256), 0, s, bt, K, *P, ws, 0,
+  hipLaunchKernelGGL(max_step_finish, grid_problems(B), dim3(256), 0, s, bt, K, *P, ws, 0,
                      out_dev);
   LAUNCH_CHECK("max_step_kernel");
   if (out4) {
@@ -1548,7 +1566,7 @@ int amigo_ipm_batch_apply_step(const amigo_ipm_batch* B, const amigo_ipm_problem
                                const double* dxlam, const double* dzl, const double* dzu,
                                const amigo_ipm_state* T, void* stream) {
   if (batch_ok(B)) return -1;
-  hipLaunchKernelGGL(apply_step_kernel, dim3(B->nblk_total), dim3(NT), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(apply_step_kernel, grid_total(B), dim3(NT), 0, (hipStream_t)stream,
                      bt_total(B), *P, *S, 0.0, 0.0, ax, az, dxlam, dzl, dzu, 0, *T);
   LAUNCH_CHECK("apply_step_kernel");
   return 0;
@@ -1561,7 +1579,7 @@ int amigo_ipm_batch_reduce(int kind, const amigo_ipm_batch* B, const amigo_ipm_p
   if (batch_ok(B)) return -1;
   hipStream_t s = (hipStream_t)stream;
   const BT bt = bt_total(B);
-  hipLaunchKernelGGL(reduce_kernel, dim3(B->nblk_total), dim3(NT), 0, s, bt, kind, *P, *S, 0.0,
+  hipLaunchKernelGGL(reduce_kernel, grid_total(B), dim3(NT), 0, s, bt, kind, *P, *S, 0.0,
                      mu, a, b, c, ws);
   LAUNCH_CHECK("reduce_kernel");
   int ops[4];
@@ -1574,7 +1592,7 @@ int amigo_ipm_batch_reset_bound_multipliers(const amigo_ipm_batch* B,
                                             const amigo_ipm_state* S, const double* mu,
                                             double kappa, void* stream) {
   if (batch_ok(B)) return -1;
-  hipLaunchKernelGGL(reset_multipliers_kernel, dim3(B->nblk_primal), dim3(NT), 0,
+  hipLaunchKernelGGL(reset_multipliers_kernel, grid_primal(B), dim3(NT), 0,
                      (hipStream_t)stream, bt_primal(B), *P, *S, 0.0, mu, kappa);
   LAUNCH_CHECK("reset_multipliers_kernel");
   return 0;
@@ -1585,9 +1603,9 @@ int amigo_ipm_batch_dual_residual_vector(const amigo_ipm_batch* B, const amigo_i
                                          double* out, void* stream) {
   if (batch_ok(B)) return -1;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(dual_residual_kernel, dim3(B->nblk_total), dim3(NT), 0, s, bt_total(B), *P,
+  hipLaunchKernelGGL(dual_residual_kernel, grid_total(B), dim3(NT), 0, s, bt_total(B), *P,
                      *S, grad, out, 0);
-  hipLaunchKernelGGL(dual_residual_fill, dim3(B->nblk_primal), dim3(NT), 0, s, bt_primal(B), *P,
+  hipLaunchKernelGGL(dual_residual_fill, grid_primal(B), dim3(NT), 0, s, bt_primal(B), *P,
                      *S, grad, out);
   LAUNCH_CHECK("dual_residual_kernel");
   return 0;
@@ -1599,7 +1617,7 @@ int amigo_ipm_batch_initialize(const amigo_ipm_batch* B, const double* lbx0,
                                const amigo_ipm_state* S, int do_relax, int do_project,
                                void* stream) {
   if (batch_ok(B)) return -1;
-  hipLaunchKernelGGL(init_kernel, dim3(B->nblk_primal), dim3(NT), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(init_kernel, grid_primal(B), dim3(NT), 0, (hipStream_t)stream,
                      bt_primal(B), lbx0, ubx0, 0, pidx, relax, cvt, kappa1, kappa2, lbx, ubx, *S,
                      do_relax, do_project);
   LAUNCH_CHECK("init_kernel");
@@ -1614,13 +1632,13 @@ int amigo_ipm_batch_floor_slacks(const amigo_ipm_batch* B, const amigo_ipm_probl
   if (batch_ok(B)) return -1;
   hipStream_t s = (hipStream_t)stream;
   const BT bt = bt_primal(B);
-  hipLaunchKernelGGL(floor_slacks_kernel, dim3(B->nblk_primal), dim3(NT), 0, s, bt, *P, *S, lbx0,
+  hipLaunchKernelGGL(floor_slacks_kernel, grid_primal(B), dim3(NT), 0, s, bt, *P, *S, lbx0,
                      ubx0, 0.0, mu, 0.0, s_min, slack_move, ws);
   LAUNCH_CHECK("floor_slacks_kernel");
   const int ops[1] = {SUM};
   const int rc = batch_finish(B, bt, 1, ops, ws, out_dev, out1, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(snap_primals_kernel, dim3(B->nblk_primal), dim3(NT), 0, s, bt, *P, *S, lbx0,
+  hipLaunchKernelGGL(snap_primals_kernel, grid_primal(B), dim3(NT), 0, s, bt, *P, *S, lbx0,
                      ubx0, out_dev);
   LAUNCH_CHECK("snap_primals_kernel");
   return 0;
@@ -1629,7 +1647,7 @@ int amigo_ipm_batch_floor_slacks(const amigo_ipm_batch* B, const amigo_ipm_probl
 int amigo_ipm_batch_scale_rows(const amigo_ipm_batch* B, const int* cidx, const double* dc,
                                double* v, int mode, void* stream) {
   if (batch_ok(B)) return -1;
-  hipLaunchKernelGGL(scatter_scale_kernel, dim3(B->nblk_total), dim3(NT), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(scatter_scale_kernel, grid_total(B), dim3(NT), 0, (hipStream_t)stream,
                      bt_total(B), 0, cidx, dc, v, mode);
   LAUNCH_CHECK("scatter_scale_kernel");
   return 0;
@@ -1641,10 +1659,10 @@ int amigo_ipm_batch_ir_init(const amigo_ipm_batch* B, const amigo_ipm_problem* P
                             double* ws, double* out_dev, double* out1, void* stream) {
   if (batch_ok(B)) return -1;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(ir_init_dz_kernel, dim3(B->nblk_primal), dim3(NT), 0, s, bt_primal(B), *P,
+  hipLaunchKernelGGL(ir_init_dz_kernel, grid_primal(B), dim3(NT), 0, s, bt_primal(B), *P,
                      *S, 0.0, mu, px, dzl, dzu);
   const BT bt = bt_total(B);
-  hipLaunchKernelGGL(ir_rhs_norm_kernel, dim3(B->nblk_total), dim3(NT), 0, s, bt, *P, *S, 0.0,
+  hipLaunchKernelGGL(ir_rhs_norm_kernel, grid_total(B), dim3(NT), 0, s, bt, *P, *S, 0.0,
                      mu, grad, rhs_saved, ws);
   LAUNCH_CHECK("ir_init");
   const int ops[1] = {MAX};
@@ -1660,7 +1678,7 @@ int amigo_ipm_batch_ir_residual(const amigo_ipm_batch* B, const amigo_ipm_proble
   if (batch_ok(B)) return -1;
   hipStream_t s = (hipStream_t)stream;
   const BT bt = bt_total(B);
-  hipLaunchKernelGGL(ir_residual_kernel, dim3(B->nblk_total), dim3(NT), 0, s, bt, *P, *S, 0.0,
+  hipLaunchKernelGGL(ir_residual_kernel, grid_total(B), dim3(NT), 0, s, bt, *P, *S, 0.0,
                      mu, grad, rhs_saved, Kpx, px, dzl, dzu, econd, ezl, ezu, ws);
   LAUNCH_CHECK("ir_residual_kernel");
   const int ops[2] = {MAX, MAX};
@@ -1672,7 +1690,7 @@ int amigo_ipm_batch_ir_update(const amigo_ipm_batch* B, const amigo_ipm_problem*
                               const double* ezu, double* px, double* dzl, double* dzu,
                               void* stream) {
   if (batch_ok(B)) return -1;
-  hipLaunchKernelGGL(ir_update_kernel, dim3(B->nblk_total), dim3(NT), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(ir_update_kernel, grid_total(B), dim3(NT), 0, (hipStream_t)stream,
                      bt_total(B), *P, *S, 0, corr, ezl, ezu, px, dzl, dzu);
   LAUNCH_CHECK("ir_update_kernel");
   return 0;
@@ -1704,7 +1722,7 @@ int amigo_segment_absmax(const double* x, const int* seg, int K, double* out_dev
 int amigo_ipm_batch_copy_state(const amigo_ipm_batch* B, const amigo_ipm_state* dst,
                                const amigo_ipm_state* src, int what, void* stream) {
   if (batch_ok(B)) return -1;
-  hipLaunchKernelGGL(copy_state_kernel, dim3(B->nblk_total), dim3(NT), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(copy_state_kernel, grid_total(B), dim3(NT), 0, (hipStream_t)stream,
                      bt_total(B), *dst, *src, what);
   LAUNCH_CHECK("copy_state_kernel");
   return 0;

@@ -95,6 +95,16 @@ int amigo_gather_sum_masked(int n, const int* ptr, const long long* src,
                             const long long* seg, int nseg, const int* seg_active,
                             void* stream);
 
+/* amigo_gather_sum_masked for a few groups of many: the outputs of the
+ * nlist groups list[0..nlist) only (device ints, each a group number g
+ * with outputs seg[g] .. seg[g+1]; max_len >= the longest of them); the
+ * other outputs keep their value and the launch's size follows the listed
+ * outputs, not n (no counterpart in the reference, which sums one
+ * problem's outputs at a time). */
+int amigo_gather_sum_listed(const int* ptr, const long long* src, const double* buf,
+                            double* out, int accumulate, const long long* seg,
+                            const int* list, int nlist, long long max_len, void* stream);
+
 /* out_dev[0] = sum of part[0..nb) in order; copied to out_host if given. */
 int amigo_sum_partials(const double* part, int nb, double* out_dev,
                        double* out_host, void* stream);

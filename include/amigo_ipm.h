@@ -204,7 +204,17 @@ int amigo_ipm_ir_update(const amigo_ipm_problem* P, const amigo_ipm_state* S,
  * (mu, tau, step lengths) are device arrays of K values.  Reductions write
  * K rows into out_dev (device, K x 4 doubles) and copy them to the host
  * array out* when it is non-NULL.  ws holds 4 x max(nblk_total,
- * nblk_primal) doubles.  All arrays in the struct are device arrays. */
+ * nblk_primal) doubles.  All arrays in the struct are device arrays.
+ *
+ * The launch lists are optional (NULL = launch every block / problem).
+ * launch_total lists nlaunch_total block numbers of the total grid (values
+ * in 0..nblk_total-1), launch_primal those of the primal grid, and
+ * launch_problems nlaunch_problems problem numbers; a call then launches
+ * only the listed blocks (for its per-block kernels) and problems (for its
+ * per-problem final passes).  They must list every block and every problem
+ * whose `active` entry is nonzero, each once; results are bitwise those of
+ * the full launch, at a cost proportional to the listed work (the last
+ * iterations of a batch, when a few problems of many are still running). */
 typedef struct {
   int num_problems;
   const int* pseg;
@@ -219,6 +229,12 @@ typedef struct {
   const int* first_primal;
   const int* count_primal;
   const int* active;
+  int nlaunch_total;
+  const int* launch_total;
+  int nlaunch_primal;
+  const int* launch_primal;
+  int nlaunch_problems;
+  const int* launch_problems;
 } amigo_ipm_batch;
 
 /* Blocks of the single-problem grid over n entries. */

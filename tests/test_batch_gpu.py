@@ -254,3 +254,99 @@ def test_batch_of_one_problem_equals_its_single_problem_solve():
     assert r["num_iterations"] == d["num_iterations"]
     assert r["inertia_history"] == d["inertia_history"]
     assert np.array_equal(bo.get_solution()[0], o.get_solution())
+
+
+def test_listed_launches_equal_masked_launches():
+    """Batched IPM kernels launched over lists of the active problems'
+    blocks (and final passes over the list of those problems) give the
+    results of the full launches with the same mask, bit for bit, and
+    leave the other problems' entries alone."""
+    from amigo_amd.batch import BatchIPM, BatchProblem, BVec
+    from amigo_amd.problems import cart_pole
+
+    probs = [cart_pole.seeded(300 + s, nmin=16, nmax=200) for s in range(12)]
+    bp = BatchProblem(probs)
+    ipm = BatchIPM(bp, bp.lower, bp.upper)
+    K, n, np_ = bp.K, bp.n, ipm.np_
+    g = torch.Generator(device="cuda").manual_seed(5)
+    R = lambda m, lo=0.1: lo + torch.rand(m, dtype=torch.float64, device="cuda", generator=g)
+    v = BVec(n, np_, "cuda", R(n, -0.5))
+    for t in (v.zl, v.zu, v.sl, v.su):
+        t.copy_(R(np_))
+    upd = BVec(n, np_, "cuda", R(n, -0.5))
+    for t in (upd.zl, upd.zu):
+        t.copy_(R(np_, -0.5))
+    grad = R(n, -0.5)
+    act = [3, 8, 9]
+    mask = np.zeros(K, np.int32)
+    mask[act] = 1
+
+    def run():
+        out = {}
+        out["abs"] = ipm.reduce("abs_sums", v, mask=mask)[act]
+        out["kkt"] = ipm.reduce("kkt_err", v, 0.1, grad, mask=mask)[act]
+        res = torch.full((n,), 7.0, dtype=torch.float64, device="cuda")
+        out["res"] = ipm.residual(0.1, v, grad, res, mask=mask)[act]
+        out["res_vec"] = res.cpu().numpy()
+        out["step"] = np.stack(ipm.max_step(0.99, v, upd, mask=mask), axis=1)[act]
+        tmp = BVec(n, np_, "cuda", torch.full((n,), 3.0, dtype=torch.float64, device="cuda"))
+        ipm.apply_step(0.5, 0.25, v, upd, tmp, mask=mask)
+        out["tmp"] = np.concatenate([t.cpu().numpy() for t in (tmp.x, tmp.zl, tmp.zu, tmp.sl, tmp.su)])
+        torch.cuda.synchronize()
+        return out
+
+    full = run()
+    # launch lists of the active problems' blocks
+    keep = []
+    for tab, (nf, cnt) in zip(ipm._tabs, (("nlaunch_total", "launch_total"), ("nlaunch_primal", "launch_primal"))):
+        first, count = tab[2].cpu().numpy(), tab[3].cpu().numpy()
+        lst = np.concatenate([first[k] + np.arange(count[k]) for k in act]).astype(np.int32)
+        d = torch.as_tensor(lst, device="cuda")
+        keep.append(d)
+        setattr(ipm._batch, nf, len(lst))
+        setattr(ipm._batch, cnt, d.data_ptr())
+    plist = torch.as_tensor(np.asarray(act, np.int32), device="cuda")
+    ipm._batch.nlaunch_problems = len(act)
+    ipm._batch.launch_problems = plist.data_ptr()
+    try:
+        listed = run()
+    finally:
+        ipm._batch.launch_total = ipm._batch.launch_primal = ipm._batch.launch_problems = None
+        ipm._batch.nlaunch_total = ipm._batch.nlaunch_primal = ipm._batch.nlaunch_problems = 0
+    for key in full:
+        assert np.array_equal(full[key].view(np.int64), listed[key].view(np.int64)), key
+
+
+def test_listed_gather_sum_equals_masked_gather_sum():
+    """amigo_gather_sum_listed writes exactly what the masked gather writes
+    for the listed groups and nothing else."""
+    import ctypes
+
+    from amigo_amd import _lib
+
+    rng = np.random.default_rng(11)
+    seg = np.concatenate([[0], np.cumsum(rng.integers(0, 900, size=9))]).astype(np.int64)
+    n = int(seg[-1])
+    cnt = rng.integers(0, 5, size=n)
+    ptr = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int32)
+    buf = rng.standard_normal(int(ptr[-1]) + 7)
+    src = rng.integers(0, len(buf), size=int(ptr[-1])).astype(np.int64)
+    act = np.array([1, 4, 5, 8], np.int32)
+    mask = np.zeros(len(seg) - 1, np.int32)
+    mask[act] = 1
+    T = lambda a: torch.as_tensor(a, device="cuda")
+    d = {k: T(a) for k, a in dict(ptr=ptr, src=src, buf=buf, seg=seg, act=act, mask=mask).items()}
+    L = _lib.lib()
+    outs = []
+    for accumulate in (0, 1):
+        o1 = T(rng.standard_normal(n))
+        o2 = o1.clone()
+        _lib.check(L.amigo_gather_sum_masked(n, d["ptr"].data_ptr(), d["src"].data_ptr(), d["buf"].data_ptr(),
+                                             o1.data_ptr(), accumulate, d["seg"].data_ptr(), len(seg) - 1,
+                                             d["mask"].data_ptr(), None), "asm")
+        _lib.check(L.amigo_gather_sum_listed(d["ptr"].data_ptr(), d["src"].data_ptr(), d["buf"].data_ptr(),
+                                             o2.data_ptr(), accumulate, d["seg"].data_ptr(), d["act"].data_ptr(),
+                                             len(act), int(np.max(np.diff(seg)[act])), None), "asm")
+        torch.cuda.synchronize()
+        assert torch.equal(o1, o2), accumulate
+        outs.append(o1)

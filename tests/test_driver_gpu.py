@@ -71,3 +71,28 @@ def test_restoration_inside_a_batch_equals_alone():
         assert d["inertia_history"] == res[k]["inertia_history"], k
         assert np.array_equal(o.get_solution(), xs[k]), k
     print(f"restored in the batch: {restored}")
+
+
+def test_last_problems_of_a_batch_equal_alone():
+    """The last problems still iterating in a batch run on launches that
+    list only their blocks and rows (a quarter of the batch or fewer);
+    their iterates equal their solves alone bit for bit, and so do those of
+    a problem that finished early."""
+    from amigo_amd.batch import BatchOptimizer
+    from amigo_amd.optimizer import Optimizer
+    from amigo_amd.problems import cart_pole
+
+    probs = [cart_pole.seeded(100 + s, nmin=16, nmax=160) for s in range(32)]
+    bo = BatchOptimizer(probs, order="ND", scaling=True)
+    res = bo.optimize({"record_objective": False})
+    xs = bo.get_solution()
+    its = np.array([r["num_iterations"] for r in res])
+    order = np.argsort(-its, kind="stable")
+    # the slowest problems run their last iterations with <= 8 of 32 active
+    assert its[order[0]] > np.sort(its)[-9], its
+    for k in list(order[:3]) + [int(order[-1])]:
+        o = Optimizer(probs[k], order="ND", scaling=True)
+        d = o.optimize({"record_objective": False})
+        assert d["num_iterations"] == res[k]["num_iterations"], k
+        assert d["inertia_history"] == res[k]["inertia_history"], k
+        assert np.array_equal(o.get_solution(), xs[k]), k
