@@ -519,9 +519,9 @@ __device__ __forceinline__ int wave_argmax_row(double c0, double c1, double mx, 
 // and after the last step.  Loads are clamped into the front so that they
 // need no exec mask; a lane stores its row's entries of the trailing
 // columns whenever the row lies in the front, entries above the diagonal
-// included: the upper triangle of an LDS front is never read (the
-// interchanges and the pivot search touch the lower triangle only) and is
-// written back as zeros.
+// included: the upper triangle of a front is never read (the interchanges
+// and the pivot search touch the lower triangle only) and is not written
+// back.
 template <int ROWS>
 __device__ __forceinline__ int factor_front_lds(double* F, int ld, int m, int nfs, int* vars,
                                                 bool bk, bool chol, double ustab,
@@ -992,10 +992,10 @@ __global__ __launch_bounds__(NT) void level_kernel(
     else
       np = factor_front_lds<2>(smem, m, m, ns + D, vars, cs == 0, chol != 0, ustab, pivot_tol,
                                cnt);
-    // The lower triangle and zeros above it.
+    // The lower triangle (nothing reads a front above its diagonal).
     for (int e = tid; e < m * m; e += NT) {
       const int j = e / m, i = e - j * m;
-      G[e] = i >= j ? smem[e] : 0.0;
+      if (i >= j) G[e] = smem[e];
     }
   } else {
     np = factor_unblocked(G, ld, m, ns + D, vars, cs == 0, chol != 0, ustab,
@@ -1043,7 +1043,8 @@ __global__ __launch_bounds__(NT) void big_zero_kernel(DevPlan P, DevState S,
       const int v = vars[j];
       dj = sc ? shift[v] * sc[v] * sc[v] : shift[v];
     }
-    for (int i = threadIdx.x; i < m; i += NT) Fj[i] = i == j ? dj : 0.0;
+    // The lower triangle only: nothing reads a front above its diagonal.
+    for (int i = j + threadIdx.x; i < m; i += NT) Fj[i] = i == j ? dj : 0.0;
   }
 }
 
