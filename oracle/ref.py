@@ -136,7 +136,8 @@ class RefLDL:
                                       delay_growth, ORDER[order])
 
     def __del__(self):
-        if getattr(self, "h", None):
+        # (at interpreter exit the module globals may already be gone)
+        if getattr(self, "h", None) and lib is not None:
             lib().ref_ldl_destroy(self.h)
             self.h = None
 
@@ -196,7 +197,7 @@ class RefCholesky:
         self.h = lib().ref_chol_create(self.n, _ip(self.rowp), _ip(self.cols), sqdef_index)
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and lib is not None:
             lib().ref_chol_destroy(self.h)
             self.h = None
 
