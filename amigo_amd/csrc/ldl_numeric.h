@@ -28,6 +28,7 @@ struct DevPlan {
   const int* aidx = nullptr;    // CSR value index
   const int* arow = nullptr;    // static row position in the front
   const int* acol = nullptr;    // column (pivot index < ns)
+  const int* acstart = nullptr; // [n+1] start of supernode column j's entries at nptr[k] + j
 };
 
 // Dynamic factor state.
@@ -207,8 +208,8 @@ class LdlNumeric {
   std::vector<int> big_nodes_;      // per level list of big fronts
   std::vector<int> big_ptr_;
   std::vector<int> big_npanels_, big_tiles_;  // static launch plan per level
-  std::vector<int> big_maxkids_;    // most children of a blocked front, per level
-  std::vector<int> big_wide_;       // level has a front too tall for the register panel
+  std::vector<int> big_mmax_;       // rows of the largest planned blocked front, per level
+  std::vector<int> big_wide_;       // level has a front too tall for the LDS panel
   std::vector<int> big_nth_;        // LDS-panel threads (= rows) of the level's launches
   std::vector<int> h_kptr_;         // children pointer (host copy)
   std::vector<int> h_m_;            // front sizes read back in sync mode

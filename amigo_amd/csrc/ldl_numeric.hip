@@ -1018,86 +1018,103 @@ __global__ __launch_bounds__(NT) void level_kernel(
 
 // ---------------------------------------------------------------------------
 // Grid-wide assembly of the blocked-path fronts of a level (the level kernel
-// only allocated them).  Three stream-ordered steps, each free of write
-// conflicts, so the sums are deterministic: (1) zero the front and place the
-// diagonal shift of the supernode pivots; (2) add the original entries;
-// (3) extend-add the children, one child index per launch (child order as in
-// sparse_ldl.h:1084-1152).
+// only allocated them), in one launch: workgroup g of a front owns the front
+// columns g, g + G, g + 2G, ... and performs every addition to their lower
+// triangle itself, in the order zero (and the diagonal shift of the
+// supernode pivots), original entries, children in child order
+// (sparse_ldl.h:1004-1156, :1084-1152).  No two workgroups write the same
+// entry, and each entry's additions come in a fixed order, so the sums are
+// deterministic.  For each child the workgroup places the child's row
+// positions in the front (pos) and their inverse (inv) in LDS; the child
+// entry (ii, jj), ii >= jj, goes to column pos[jj] when pos[ii] >= pos[jj]
+// (read down child column jj) and otherwise, transposed, to column pos[ii]
+// (read along child row ii; only a child's delayed rows, placed after the
+// supernode rows, land there).  Dynamic LDS: 2 * lds_ints ints.
 __device__ __forceinline__ bool big_active(const DevState& S, int node) {
   return node_on(S, node) && S.pstate[PS * node + 4] == 1 && fail_flag(S) == 0;
 }
 
-__global__ __launch_bounds__(NT) void big_zero_kernel(DevPlan P, DevState S,
-                                                      const int* __restrict__ bnodes,
-                                                      const double* __restrict__ shift,
-                                                      const double* __restrict__ sc) {
-  const int node = bnodes[blockIdx.y];
-  if (!big_active(S, node)) return;
-  const int m = S.m[node], ns = P.ns[node];
-  double* F = S.rpool + S.foff[node];
-  const int* vars = S.ipool + S.voff[node];
-  for (int j = blockIdx.x; j < m; j += gridDim.x) {
-    double* Fj = F + (size_t)j * m;
-    double dj = 0.0;
-    if (j < ns && shift != nullptr) {
-      const int v = vars[j];
-      dj = sc ? shift[v] * sc[v] * sc[v] : shift[v];
-    }
-    // The lower triangle only: nothing reads a front above its diagonal.
-    for (int i = j + threadIdx.x; i < m; i += NT) Fj[i] = i == j ? dj : 0.0;
-  }
-}
-
-__global__ __launch_bounds__(NT) void big_orig_kernel(DevPlan P, DevState S,
-                                                      const int* __restrict__ bnodes,
-                                                      const double* __restrict__ vals,
-                                                      const double* __restrict__ sc) {
+__global__ __launch_bounds__(NT) void big_assemble_kernel(DevPlan P, DevState S,
+                                                          const int* __restrict__ bnodes,
+                                                          const double* __restrict__ vals,
+                                                          const double* __restrict__ shift,
+                                                          const double* __restrict__ sc,
+                                                          int lds_ints) {
+  extern __shared__ int smap[];
+  int* inv = smap;            // front position -> child row (or -1)
+  int* pos = smap + lds_ints; // child row -> front position
   const int node = bnodes[blockIdx.y];
   if (!big_active(S, node)) return;
   const int m = S.m[node], ns = P.ns[node], D = S.D[node];
+  const int tid = threadIdx.x, G = gridDim.x, g = blockIdx.x;
+  if (m > lds_ints) {
+    // More rows than the launch was sized for: redo the factorization with
+    // per-level sizes.
+    if (g == 0 && tid == 0) atomicExch(&S.flags[0], -7);
+    return;
+  }
   double* F = S.rpool + S.foff[node];
   const int* vars = S.ipool + S.voff[node];
-  for (int t = P.aptr[node] + blockIdx.x * NT + threadIdx.x; t < P.aptr[node + 1];
-       t += gridDim.x * NT) {
-    int r = P.arow[t];
-    r = r < ns ? r : r + D;
-    const int c = P.acol[t];
-    double v = vals[P.aidx[t]];
-    if (sc) v *= sc[vars[r]] * sc[vars[c]];
-    if (r >= c)
-      F[r + (size_t)c * m] += v;
-    else
-      F[c + (size_t)r * m] += v;
+  const int vb = P.nptr[node];
+  // Zero, with the diagonal shift of the supernode pivots.
+  for (int c = g; c < m; c += G) {
+    double* Fc = F + (size_t)c * m;
+    double dc = 0.0;
+    if (c < ns && shift != nullptr) {
+      const int v = vars[c];
+      dc = sc ? shift[v] * sc[v] * sc[v] : shift[v];
+    }
+    for (int i = c + tid; i < m; i += NT) Fc[i] = i == c ? dc : 0.0;
   }
-}
-
-__global__ __launch_bounds__(NT) void big_child_kernel(DevPlan P, DevState S,
-                                                       const int* __restrict__ bnodes,
-                                                       int q) {
-  const int node = bnodes[blockIdx.y];
+  __syncthreads();
+  // Original entries of the supernode columns (each on or below the
+  // diagonal of its column).
+  for (int c = g; c < ns; c += G) {
+    double* Fc = F + (size_t)c * m;
+    const int sc_c = sc ? vars[c] : 0;
+    for (int t = P.acstart[vb + c] + tid; t < P.acstart[vb + c + 1]; t += NT) {
+      int r = P.arow[t];
+      r = r < ns ? r : r + D;
+      double v = vals[P.aidx[t]];
+      if (sc) v *= sc[vars[r]] * sc[sc_c];
+      Fc[r] += v;
+    }
+  }
+  // Children, in child order.
   const int kb = P.kptr[node], ke = P.kptr[node + 1];
-  if (kb + q >= ke || !big_active(S, node)) return;
-  const int m = S.m[node], ns = P.ns[node], D = S.D[node];
-  double* F = S.rpool + S.foff[node];
   int doff = ns;
-  for (int t = kb; t < kb + q; t++) doff += S.ndel[P.kids[t]];
-  const int c = P.kids[kb + q];
-  const int pc = S.npiv[c], dc = S.ndel[c], mc = S.m[c];
-  const int w = mc - pc;
-  const double* Cf = S.rpool + S.foff[c];
-  for (int jj = blockIdx.x; jj < w; jj += gridDim.x) {
-    const int pj = child_row_pos(P, c, dc, doff, ns, D, jj);
-    const double* Cj = Cf + pc + (size_t)(pc + jj) * mc;
-    for (int ii = jj + threadIdx.x; ii < w; ii += NT) {
-      const int pi = child_row_pos(P, c, dc, doff, ns, D, ii);
-      const double v = Cj[ii];
-      if (pi >= pj)
-        F[pi + (size_t)pj * m] += v;
-      else
-        F[pj + (size_t)pi * m] += v;
+  for (int q = kb; q < ke; q++) {
+    const int ch = P.kids[q];
+    const int pc = S.npiv[ch], dc = S.ndel[ch], mc = S.m[ch];
+    const int w = mc - pc;
+    __syncthreads();  // the previous child's additions and maps are done
+    for (int i = tid; i < m; i += NT) inv[i] = -1;
+    __syncthreads();
+    for (int ii = tid; ii < w; ii += NT) {
+      const int p = child_row_pos(P, ch, dc, doff, ns, D, ii);
+      pos[ii] = p;
+      inv[p] = ii;
     }
+    __syncthreads();
+    const double* Cf = S.rpool + S.foff[ch] + pc + (size_t)pc * mc;  // C[ii, jj] = Cf[ii + jj mc]
+    for (int c = g; c < m; c += G) {
+      const int x = inv[c];
+      if (x < 0) continue;
+      double* Fc = F + (size_t)c * m;
+      // Down child column x: rows whose position is at or below c.
+      const double* Cx = Cf + (size_t)x * mc;
+      for (int ii = x + tid; ii < w; ii += NT) {
+        const int pi = pos[ii];
+        if (pi >= c) Fc[pi] += Cx[ii];
+      }
+      // Along child row x: earlier child columns placed below row c.
+      for (int jj = tid; jj < x; jj += NT) {
+        const int pj = pos[jj];
+        if (pj > c) Fc[pj] += Cf[x + (size_t)jj * mc];
+      }
+    }
+    doff += dc;
   }
-  (void)m;
 }
 
 // s - sum_q Fi[q * ld] * wr[q] for q < kw, four independent partial sums so
@@ -2533,10 +2550,14 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
 
   // Assembly map of original entries (sparse_ldl.h:1004-1081): column j of
   // the supernode takes rows with permuted index >= its own.
-  std::vector<int> aptr(K + 1, 0), aidx, arow, acol;
+  // Original entries of each supernode column j (all on or below the
+  // diagonal, so their front column is j), column by column: acstart gives
+  // each column's first entry.
+  std::vector<int> aptr(K + 1, 0), aidx, arow, acol, acstart(n_ + 1, 0);
   for (int k = 0; k < K; k++) {
     for (int j = 0; j < cs[k]; j++) cpos[plan.contrib_rows[plan.contrib_ptr[k] + j]] = j;
     for (int j = 0; j < ns[k]; j++) {
+      acstart[plan.node_ptr[k] + j] = int(aidx.size());
       int var = plan.node_to_var[plan.node_ptr[k] + j];
       int pj = pinv(var);
       for (int ip = rowp[var]; ip < rowp[var + 1]; ip++) {
@@ -2556,6 +2577,7 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
     for (int j = 0; j < cs[k]; j++) cpos[plan.contrib_rows[plan.contrib_ptr[k] + j]] = -1;
     aptr[k + 1] = int(aidx.size());
   }
+  acstart[n_] = int(aidx.size());
 
   // Levels by height (children have smaller indices in the postorder).
   std::vector<int> level(K, 0);
@@ -2618,8 +2640,8 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   big_nodes_.clear();
   big_npanels_.assign(nlev, 0);
   big_tiles_.assign(nlev, 0);
-  big_maxkids_.assign(nlev, 0);
   big_wide_.assign(nlev, 0);
+  big_mmax_.assign(nlev, 0);
   h_kptr_ = plan.children_ptr;
   std::vector<int> big_root, big_ns;
   std::vector<int64_t> big_woff;
@@ -2639,10 +2661,9 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
         big_woff.push_back(wtot);
         wtot += (int64_t)mcap * NBW;
         big_npanels_[l] = std::max(big_npanels_[l], (ns[k] + DPAN_BIG + NBR - 1) / NBR);
-        big_maxkids_[l] = std::max(big_maxkids_[l],
-                                   plan.children_ptr[k + 1] - plan.children_ptr[k]);
         if (mcap > NTP) big_wide_[l] = 1;
         big_mcap = std::max(big_mcap, mcap);
+        big_mmax_[l] = std::max(big_mmax_[l], mcap);
         big_tiles_[l] = std::max(big_tiles_[l], (mcap + TILE - 1) / TILE);
       }
       if (t >= level_split_[l]) mx = std::max(mx, std::min(ns[k] + cs[k] + 32, LDS_MAX_M));
@@ -2668,6 +2689,7 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   if (upload(allocs_, aidx, &P_.aidx, err)) return -2;
   if (upload(allocs_, arow, &P_.arow, err)) return -2;
   if (upload(allocs_, acol, &P_.acol, err)) return -2;
+  if (upload(allocs_, acstart, &P_.acstart, err)) return -2;
   {
     std::vector<int> rp(rowp, rowp + n + 1), cl(cols, cols + rowp[n]);
     if (upload(allocs_, rp, &d_rowp_, err)) return -2;
@@ -2735,6 +2757,9 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
       HIPCHK(hipFuncSetAttribute((const void*)panel_kernel<true, NTP / 4>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)PANEL_LDS_BYTES / 4));
+      HIPCHK(hipFuncSetAttribute((const void*)big_assemble_kernel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 150 * 1024));
       HIPCHK(hipFuncSetAttribute((const void*)solve_fwd_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  150 * 1024));
@@ -2905,26 +2930,22 @@ int LdlNumeric::enqueue_factor(const double* vals, const double* shift,
       woff = d_big_woff_sync_;
     }
     {
-      int maxkids = 0;
+      // Rows of the level's largest blocked front (planned, or level-
+      // synchronously the actual ones): the assembly's position maps.
+      int mmax = big_mmax_[l];
       if (sync_mode) {
+        mmax = 0;
         for (int t = level_ptr_[l]; t < level_ptr_[l + 1]; t++) {
           const int node = level_nodes_[t];
           if (nbatch_ && !h_blk_active_[h_node_blk_[node]]) continue;
-          if (h_m_[node] > LDS_MAX_M)
-            maxkids = std::max(maxkids, h_kptr_[node + 1] - h_kptr_[node]);
+          if (h_m_[node] > LDS_MAX_M) mmax = std::max(mmax, h_m_[node]);
         }
-      } else {
-        maxkids = big_maxkids_[l];
       }
       const double* scp = scaling_ ? scale_ : nullptr;
       tk_begin(stream);
-      hipLaunchKernelGGL(big_zero_kernel, dim3(128, nlaunch), dim3(NT), 0, stream, P_, S_,
-                         bnodes, shift, scp);
-      hipLaunchKernelGGL(big_orig_kernel, dim3(16, nlaunch), dim3(NT), 0, stream, P_, S_,
-                         bnodes, vals, scp);
-      for (int q = 0; q < maxkids; q++)
-        hipLaunchKernelGGL(big_child_kernel, dim3(128, nlaunch), dim3(NT), 0, stream, P_,
-                           S_, bnodes, q);
+      hipLaunchKernelGGL(big_assemble_kernel, dim3(128, nlaunch), dim3(NT),
+                         2 * sizeof(int) * (size_t)mmax, stream, P_, S_, bnodes, vals, shift,
+                         scp, mmax);
       tk_end(stream, TK_ASSEMBLE);
       HIPCHK(hipGetLastError());
     }
