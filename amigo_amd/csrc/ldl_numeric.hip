@@ -527,7 +527,9 @@ __device__ __forceinline__ int factor_front_lds(double* F, int ld, int m, int nf
                                                 bool bk, bool chol, double ustab,
                                                 double pivot_tol, int* cnt) {
   static_assert(ROWS == 1 || ROWS == 2, "one or two rows per lane");
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // The wave index as a scalar: the update's column indices, bounds and
+  // readlane positions derive from it and stay in scalar registers.
+  const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   constexpr int NW = NT / 64;
   const int mlast = m - 1;
   int k = 0, nc = nfs;
@@ -1780,7 +1782,7 @@ __global__ __launch_bounds__(NT) void trailing_mfma_kernel(DevState S,
     }
   }
   __syncthreads();
-  const int lane = tid & 63, wv = tid >> 6;
+  const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // scalar
   const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
   V4 acc[2][2];
