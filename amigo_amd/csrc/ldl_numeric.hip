@@ -642,38 +642,42 @@ __device__ __forceinline__ int factor_front_lds(double* F, int ld, int m, int nf
       }
     }
     pk = -1;
-    // The trailing lower triangle from this block's unscaled columns: a
-    // wave per column, lanes down the rows, the loads of UC columns in
-    // flight together.
+    // The trailing lower triangle from this block's unscaled columns:
+    // each wave a contiguous block of columns, lanes down the rows, the
+    // loads of UC columns in flight together; the pivot columns' entries at
+    // row j are uniform LDS reads (broadcast), the column bounds scalar.
     constexpr int UC = 8;
-    // Value of a lane-held column at row j >= k (uniform j).
-    auto at_row = [&](double x0, double x1, int j) {
-      const int t = j - k;
-      return lane_d(ROWS == 2 && t >= 64 ? x1 : x0, t & 63);
+    auto wave_cols = [&](int j0, int& ja, int& je) {
+      const int per = (m - j0 + NW - 1) / NW;
+      ja = j0 + wv * per;
+      je = min(ja + per, m);
     };
     if (step == 1) {
       const double d = F[k + k * ld];
       if (!zero) {
         const double dinv = 1.0 / d;
         const double l0 = v0 * dinv, l1 = v1 * dinv;
-        for (int jb = k + 1 + wv; jb < m; jb += NW * UC) {
+        const double* ck = F + k * ld;
+        int ja, je;
+        wave_cols(k + 1, ja, je);
+        for (int jb = ja; jb < je; jb += UC) {
           double cj[UC], f0[UC], f1[UC];
 #pragma unroll
           for (int u = 0; u < UC; u++) {
-            const int j = min(jb + u * NW, mlast);
-            cj[u] = at_row(v0, v1, j);
+            const int j = min(jb + u, je - 1);
+            cj[u] = ck[j];
             f0[u] = F[c0 + j * ld];
             f1[u] = ROWS == 2 ? F[c1 + j * ld] : 0.0;
           }
           if (r0 < m) {
 #pragma unroll
             for (int u = 0; u < UC; u++)
-              if (jb + u * NW < m) F[r0 + (jb + u * NW) * ld] = f0[u] - l0 * cj[u];
+              if (jb + u < je) F[r0 + (jb + u) * ld] = f0[u] - l0 * cj[u];
           }
           if (ROWS == 2 && r1 < m) {
 #pragma unroll
             for (int u = 0; u < UC; u++)
-              if (jb + u * NW < m) F[r1 + (jb + u * NW) * ld] = f1[u] - l1 * cj[u];
+              if (jb + u < je) F[r1 + (jb + u) * ld] = f1[u] - l1 * cj[u];
           }
         }
         pk = k;
@@ -692,27 +696,29 @@ __device__ __forceinline__ int factor_front_lds(double* F, int ld, int m, int nf
       const double u0 = F[c0 + (k + 1) * ld], u1 = ROWS == 2 ? F[c1 + (k + 1) * ld] : 0.0;
       const double p0 = v0 * i00 + u0 * i01, s0 = v0 * i01 + u0 * i11;
       const double p1 = v1 * i00 + u1 * i01, s1 = v1 * i01 + u1 * i11;
-      for (int jb = k + 2 + wv; jb < m; jb += NW * UC) {
+      const double* ck = F + k * ld;
+      const double* ck1 = F + (k + 1) * ld;
+      int ja, je;
+      wave_cols(k + 2, ja, je);
+      for (int jb = ja; jb < je; jb += UC) {
         double a1[UC], a2[UC], f0[UC], f1[UC];
 #pragma unroll
         for (int u = 0; u < UC; u++) {
-          const int j = min(jb + u * NW, mlast);
-          a1[u] = at_row(v0, v1, j);
-          a2[u] = at_row(u0, u1, j);
+          const int j = min(jb + u, je - 1);
+          a1[u] = ck[j];
+          a2[u] = ck1[j];
           f0[u] = F[c0 + j * ld];
           f1[u] = ROWS == 2 ? F[c1 + j * ld] : 0.0;
         }
         if (r0 < m) {
 #pragma unroll
           for (int u = 0; u < UC; u++)
-            if (jb + u * NW < m)
-              F[r0 + (jb + u * NW) * ld] = f0[u] - (p0 * a1[u] + s0 * a2[u]);
+            if (jb + u < je) F[r0 + (jb + u) * ld] = f0[u] - (p0 * a1[u] + s0 * a2[u]);
         }
         if (ROWS == 2 && r1 < m) {
 #pragma unroll
           for (int u = 0; u < UC; u++)
-            if (jb + u * NW < m)
-              F[r1 + (jb + u * NW) * ld] = f1[u] - (p1 * a1[u] + s1 * a2[u]);
+            if (jb + u < je) F[r1 + (jb + u) * ld] = f1[u] - (p1 * a1[u] + s1 * a2[u]);
         }
       }
       pk = k;
