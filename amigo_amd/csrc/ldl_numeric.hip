@@ -181,6 +181,12 @@ struct PivotChoice {
 
 constexpr double BK_ALPHA = 0.6403882032022076;  // (1 + sqrt(17)) / 8
 
+// A value every thread of the workgroup holds alike (the pivot decisions
+// are taken by all threads from the same data), made scalar for the
+// compiler: loop bounds, predicates and indices derived from it then live
+// in scalar registers and branch without exec masks.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // Inertia of a 1x1 / 2x2 pivot block (sparse_ldl.h:1917-1969): zero counts
 // as positive like the reference; `zero` is reported separately.
 __device__ void count_inertia(int step, double d11, double d21, double d22,
@@ -532,7 +538,7 @@ __device__ __forceinline__ int factor_front_lds(double* F, int ld, int m, int nf
   const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   constexpr int NW = NT / 64;
   const int mlast = m - 1;
-  int k = 0, nc = nfs;
+  int k = 0, nc = uni(nfs);
   // The pivot block whose columns still hold the unscaled values: its rows
   // to scale start at the current k.
   int pk = -1, pstep = 0;
@@ -609,6 +615,9 @@ __device__ __forceinline__ int factor_front_lds(double* F, int ld, int m, int nf
         }
       }
     }
+    step = uni(step);
+    kpiv = uni(kpiv);
+    zero = uni(zero) != 0;
     if (step == 0) {
       // symm_swap_lds starts with a barrier: every wave has read what it moves.
       if (k < nc - 1) symm_swap_lds(F, ld, m, vars, k, nc - 1);
@@ -1338,6 +1347,9 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
     if (wlane >= 0 && wlane < kw && k < m) sh.wrow[buf][wlane] = W[k + (size_t)wlane * ldw];
   };
   int wb = 0;
+  k = uni(k);
+  nc = uni(nc);
+  kw = uni(kw);
   // The first pivot row's W row is empty (kw = 0 at the panel's start).
   while (kw < NBR && k < nc) {
     const double* wr = sh.wrow[wb];
@@ -1448,6 +1460,10 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
         }
       }
     }
+    step = uni(step);
+    kpiv = uni(kpiv);
+    zero = uni(zero) != 0;
+    plain = uni(plain) != 0;
 
     if (step == 0) {
       // The candidate goes to the end of the candidates (sparse_ldl.h:1441).
