@@ -305,6 +305,9 @@ __device__ int factor_unblocked(double* F, int ld, int m, int nfs, int* vars,
         }
       }
     }
+    step = uni(step);
+    kpiv = uni(kpiv);
+    zero = uni(zero) != 0;
 
     if (step == 0) {
       if (k < nc - 1) symm_swap(F, ld, m, vars, k, nc - 1);
@@ -858,7 +861,7 @@ __global__ __launch_bounds__(NT) void level_kernel(
   }
   __syncthreads();
   if (sh[0] < 0) return;
-  const int D = sh[0], m = sh[1];
+  const int D = uni(sh[0]), m = uni(sh[1]);
   double* G = S.rpool + S.foff[node];
   int* vars = S.ipool + S.voff[node];
   // Fronts larger than the LDS limit go to the blocked path (decided on the
@@ -1559,11 +1562,11 @@ __global__ __launch_bounds__(REG ? NTH : NTP) void panel_kernel(DevState S,
   if (!node_on(S, node)) return;
   int* ps = S.pstate + PS * node;
   if (ps[4] != 1) return;  // factored in a workgroup of the level kernel
-  const int m = S.m[node];
+  const int m = uni(S.m[node]);
   double* F = S.rpool + S.foff[node];
   int* vars = S.ipool + S.voff[node];
   const int ld = m, ldw = m;
-  int k = ps[0], nc = ps[1];
+  int k = uni(ps[0]), nc = uni(ps[1]);
   if (tid == 0) cnt[0] = cnt[1] = cnt[2] = cnt[3] = 0;
   __syncthreads();
   if (k >= nc) {
@@ -1678,6 +1681,9 @@ __global__ __launch_bounds__(REG ? NTH : NTP) void panel_kernel(DevState S,
         step = 0;
       }
     }
+    step = uni(step);
+    kpiv = uni(kpiv);
+    zero = uni(zero) != 0;
 
     if (step == 0) {
       if (k < nc - 1) {
@@ -1772,8 +1778,8 @@ __global__ __launch_bounds__(NT) void trailing_mfma_kernel(DevState S,
   if (bad != 0) return;
   if (!node_on(S, node)) return;
   const int* ps = S.pstate + PS * node;
-  const int m = S.m[node];
-  const int k = ps[0], kstart = ps[2], kw = ps[3];
+  const int m = uni(S.m[node]);
+  const int k = uni(ps[0]), kstart = uni(ps[2]), kw = uni(ps[3]);
   const int ti = blockIdx.x, tj = blockIdx.y;
   if (kw == 0 || ti < tj) return;
   const int i0 = k + ti * TILE, j0 = k + tj * TILE;
