@@ -1788,6 +1788,23 @@ __global__ __launch_bounds__(NT) void trailing_mfma_kernel(DevState S,
   const int ld = m, ldw = m;
   const int tid = threadIdx.x;
   const int kp = (kw + 3) & ~3;
+  const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // scalar
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  // This thread's 16 entries of the tile, read first so that their latency
+  // overlaps the operand loads (the tile lies in columns >= k, the operands
+  // in the panel's columns and in W).
+  double old[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) {
+        const int i = i0 + wr + a * 16 + (sizeof(T) == 8 ? lk + 4 * jj : lk * 4 + jj);
+        const int j = j0 + wc + b * 16 + lr;
+        old[a][b][jj] = (i < m && j < m && i >= j) ? F[i + (size_t)j * ld] : 0.0;
+      }
   // Operand tiles into LDS, 2 x 8 loads per thread in flight.
   const int tot = TILE * kp;
   for (int e0 = tid; e0 < tot; e0 += 8 * NT) {
@@ -1810,9 +1827,6 @@ __global__ __launch_bounds__(NT) void trailing_mfma_kernel(DevState S,
     }
   }
   __syncthreads();
-  const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // scalar
-  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
-  const int lr = lane & 15, lk = lane >> 4;
   V4 acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; a++)
@@ -1829,19 +1843,6 @@ __global__ __launch_bounds__(NT) void trailing_mfma_kernel(DevState S,
 #pragma unroll
       for (int b = 0; b < 2; b++) acc[a][b] = mfma_16x16x4(av[a], bv[b], acc[a][b]);
   }
-  // Read all 16 old entries first (stores to F could alias later loads, so
-  // the compiler would otherwise wait on each load in turn).
-  double old[2][2][4];
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++)
-#pragma unroll
-      for (int jj = 0; jj < 4; jj++) {
-        const int i = i0 + wr + a * 16 + (sizeof(T) == 8 ? lk + 4 * jj : lk * 4 + jj);
-        const int j = j0 + wc + b * 16 + lr;
-        old[a][b][jj] = (i < m && j < m && i >= j) ? F[i + (size_t)j * ld] : 0.0;
-      }
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
