@@ -211,6 +211,7 @@ class LdlNumeric {
   std::vector<int> big_mmax_;       // rows of the largest planned blocked front, per level
   std::vector<int> big_wide_;       // level has a front too tall for the LDS panel
   std::vector<int> big_nth_;        // LDS-panel threads (= rows) of the level's launches
+  std::vector<int> solve_wide_;     // level solved by 1,024-thread workgroups
   std::vector<int> h_kptr_;         // children pointer (host copy)
   std::vector<int> h_m_;            // front sizes read back in sync mode
   const int* d_big_nodes_ = nullptr;

@@ -2044,23 +2044,29 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 // Pivot blocks L[0:p, 0:p] (p <= 64) are staged in LDS as packed lower
 // triangles: column j starts at tri_off(j, p).
 constexpr int PIVOT_TRI = 64 * 65 / 2;
+// Solve workgroups: NT threads, or NTS on levels of at most SOLVE_WIDE_FRONTS
+// fronts with one of at least SOLVE_WIDE_ROWS rows.
+constexpr int NTS = 1024;
+constexpr int SOLVE_WIDE_FRONTS = 256;
+constexpr int SOLVE_WIDE_ROWS = 256;
 __device__ __forceinline__ int tri_off(int j, int p) { return j * p - ((j * (j - 1)) >> 1); }
 
 // All threads; 16 loads per thread are in flight before any is stored.
 // Ends with a barrier.
+template <int NTH>
 __device__ __forceinline__ void stage_pivot_block(const double* __restrict__ L, int ld, int p,
                                                   double* Ls) {
   const int tot = p * p;
-  for (int e0 = threadIdx.x; e0 < tot; e0 += 16 * NT) {
+  for (int e0 = threadIdx.x; e0 < tot; e0 += 16 * NTH) {
     double a[16];
 #pragma unroll
     for (int u = 0; u < 16; u++) {
-      const int e = e0 + u * NT, j = e / p, i = e - j * p;
+      const int e = e0 + u * NTH, j = e / p, i = e - j * p;
       a[u] = (e < tot && i >= j) ? L[i + (size_t)j * ld] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < 16; u++) {
-      const int e = e0 + u * NT, j = e / p, i = e - j * p;
+      const int e = e0 + u * NTH, j = e / p, i = e - j * p;
       if (e < tot && i >= j) Ls[tri_off(j, p) + i - j] = a[u];
     }
   }
@@ -2135,13 +2141,14 @@ __device__ __forceinline__ double wave_upper_solve(double tl, int l, int p, cons
 // groups (columns g, g + G, ...), with 16 loads in flight; the G partial sums
 // of a row are added in group order, so the result is deterministic.  red
 // holds NT doubles of LDS.  All threads; R = m - p is uniform.
+template <int NTH>
 __device__ __forceinline__ void front_update(const double* __restrict__ L, int ld, int p, int m,
                                              const double* t, double* __restrict__ uo,
                                              double* red) {
   const int tid = threadIdx.x, R = m - p;
   if (R <= 0) return;
-  if (R < NT) {
-    const int G = max(1, min(NT / R, p));
+  if (R < NTH) {
+    const int G = max(1, min(NTH / R, p));
     double s = 0.0;
     if (tid < G * R) {
       const int g = tid / R, i = p + tid - g * R;
@@ -2168,7 +2175,7 @@ __device__ __forceinline__ void front_update(const double* __restrict__ L, int l
       uo[p + tid] = a;
     }
   } else {
-    for (int i = p + tid; i < m; i += NT) {
+    for (int i = p + tid; i < m; i += NTH) {
       const double* Li = L + i;
       double s = 0.0;
       for (int j0 = 0; j0 < p; j0 += 16) {
@@ -2212,15 +2219,60 @@ __device__ __forceinline__ double wave_reduce16(double (&v)[16], int lane) {
 }
 
 // Front update of the backward solve: t[j] -= sum_{i in [p,m)} L[i, j] t[i]
-// for j < p.  Wave w takes columns jb + w + NW u (u < 16) of each block of
-// 16 NW columns; its lanes run over the rows 64 at a time with 16 loads in
-// flight.  All threads; ends with a barrier.
+// for j < p.  With at least as many 16-column blocks as waves, wave w takes
+// columns jb + w + NW u (u < 16) of each block of 16 NW columns and its
+// lanes run over all the rows 64 at a time.  With more waves than column
+// blocks (few pivots, a 1024-thread workgroup), the RS = NW / CB waves of a
+// block split its rows (wave rs of the block takes row groups rs, rs + RS,
+// ...) and their sums are added in rs order through red (NW * 16 doubles of
+// LDS).  16 loads in flight per lane; deterministic.  All threads; ends
+// with a barrier.
+template <int NTH>
 __device__ __forceinline__ void front_update_t(const double* __restrict__ L, int ld, int p,
-                                               int m, double* t) {
-  constexpr int NW = NT / 64;
+                                               int m, double* t, double* red) {
+  constexpr int NW = NTH / 64;
   const int lane = threadIdx.x & 63;
   // The wave index is uniform: the column addresses stay in scalar registers.
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int CB = (p + 15) >> 4;
+  if (NTH >= 1024 && NW >= 2 * CB) {
+    const int RS = NW / CB;
+    const int cb = w / RS, rs = w - cb * RS;
+    if (cb < CB) {
+      double s[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) s[u] = 0.0;
+      // Columns past p read column p - 1 (in bounds) and are dropped below.
+      const int ju = min(15, p - 1 - 16 * cb);
+      const double* Lc = L + (size_t)(16 * cb) * ld;
+      for (unsigned i = p + 64 * rs + lane; i < (unsigned)m; i += 64 * RS) {
+        const double ti = t[i];
+        double a[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          const double* cu = Lc + __builtin_amdgcn_readfirstlane(min(u, ju) * ld);
+          a[u] = cu[i];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++) s[u] += a[u] * ti;
+      }
+      const double tot = wave_reduce16(s, lane);
+      if ((lane & 3) == 0) {
+        const int u = 8 * ((lane >> 5) & 1) + 4 * ((lane >> 4) & 1) + 2 * ((lane >> 3) & 1) +
+                      ((lane >> 2) & 1);
+        red[(rs * CB + cb) * 16 + u] = tot;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < p) {
+      const int j = threadIdx.x, cb = j >> 4, u = j & 15;
+      double tot = 0.0;
+      for (int r = 0; r < RS; r++) tot += red[(r * CB + cb) * 16 + u];
+      t[j] -= tot;
+    }
+    __syncthreads();
+    return;
+  }
   for (int jb = 0; jb < p; jb += 16 * NW) {
     if (jb + w >= p) continue;  // no column of this block for this wave
     double s[16];
@@ -2261,7 +2313,8 @@ __device__ __forceinline__ void front_update_t(const double* __restrict__ L, int
 // stored for the parent, then t = D^-1 t goes to x.  Update vectors live in
 // ub at the front's index-pool offset (ld of one right-hand side = ubld).
 // Dynamic LDS: t (tcap >= m doubles), the packed pivot block, NT doubles.
-__global__ __launch_bounds__(NT) void solve_fwd_kernel(DevPlan P, DevState S,
+template <int NTH>
+__global__ __launch_bounds__(NTH) void solve_fwd_kernel(DevPlan P, DevState S,
                                                        const int* __restrict__ lnodes,
                                                        double* __restrict__ x,
                                                        int ldx, int nrhs,
@@ -2282,11 +2335,11 @@ __global__ __launch_bounds__(NT) void solve_fwd_kernel(DevPlan P, DevState S,
   const int kb = P.kptr[node], ke = P.kptr[node + 1];
   const int* fpos = S.fpos + S.voff[node];
   const bool wave = p <= 64;
-  if (wave) stage_pivot_block(L, ld, p, Ls);
+  if (wave) stage_pivot_block<NTH>(L, ld, p, Ls);
   for (int r = 0; r < nrhs; r++) {
     double* xr = x + (size_t)r * ldx;
     double* ur = ub + (size_t)r * ubld;
-    for (int j = tid; j < m; j += NT) {
+    for (int j = tid; j < m; j += NTH) {
       double v = 0.0;
       if (j < p) {
         const int q = vars[j];
@@ -2296,7 +2349,7 @@ __global__ __launch_bounds__(NT) void solve_fwd_kernel(DevPlan P, DevState S,
     }
     __syncthreads();
     int doff = ns;
-    for (int q0 = kb; q0 < ke; q0 += NT) {
+    for (int q0 = kb; q0 < ke; q0 += NT) {  // children in batches of NT
       const int nb = min(NT, ke - q0);
       if (tid < nb) {
         const int c = P.kids[q0 + tid];
@@ -2311,7 +2364,7 @@ __global__ __launch_bounds__(NT) void solve_fwd_kernel(DevPlan P, DevState S,
         const int dc = kmeta[0][qq], wc = kmeta[1][qq];
         const double* uc = ur + kmeta[2][qq];
         const int* cm = P.cmap + kmeta[3][qq] - dc;  // cm[ii] for ii >= dc
-        for (int ii = tid; ii < wc; ii += NT) {
+        for (int ii = tid; ii < wc; ii += NTH) {
           int pos;
           if (ii < dc) {
             pos = doff + ii;
@@ -2335,7 +2388,7 @@ __global__ __launch_bounds__(NT) void solve_fwd_kernel(DevPlan P, DevState S,
         if (tid < p) t[tid] = tl;
       }
       __syncthreads();
-      front_update(L, ld, p, m, t, uo, red);
+      front_update<NTH>(L, ld, p, m, t, uo, red);
       if (tid < 64) {
         tl = wave_d_solve(tl, tid, p, role, Ls);
         if (tid < p) {
@@ -2349,17 +2402,17 @@ __global__ __launch_bounds__(NT) void solve_fwd_kernel(DevPlan P, DevState S,
     for (int j = 0; j < p;) {
       if (vars[j] >= 0) {
         const double tj = t[j];
-        for (int i = j + 1 + tid; i < p; i += NT) t[i] -= L[i + (size_t)j * ld] * tj;
+        for (int i = j + 1 + tid; i < p; i += NTH) t[i] -= L[i + (size_t)j * ld] * tj;
         j += 1;
       } else {
         const double t1 = t[j], t2 = t[j + 1];
-        for (int i = j + 2 + tid; i < p; i += NT)
+        for (int i = j + 2 + tid; i < p; i += NTH)
           t[i] -= L[i + (size_t)j * ld] * t1 + L[i + (size_t)(j + 1) * ld] * t2;
         j += 2;
       }
       __syncthreads();
     }
-    front_update(L, ld, p, m, t, uo, red);
+    front_update<NTH>(L, ld, p, m, t, uo, red);
     __syncthreads();  // t[0:p) is read above and rescaled below
     if (tid == 0) {
       for (int i = 0; i < p;) {
@@ -2378,7 +2431,7 @@ __global__ __launch_bounds__(NT) void solve_fwd_kernel(DevPlan P, DevState S,
       }
     }
     __syncthreads();
-    for (int j = tid; j < p; j += NT) {
+    for (int j = tid; j < p; j += NTH) {
       int v = vars[j];
       xr[v >= 0 ? v : -v - 1] = t[j];
     }
@@ -2387,8 +2440,9 @@ __global__ __launch_bounds__(NT) void solve_fwd_kernel(DevPlan P, DevState S,
 }
 
 // Backward: t = x[piv]; t -= L21^T x[rows below]; L11^T solve; x[piv] = t.
-// Dynamic LDS: t (tcap >= m doubles) then the packed pivot block.
-__global__ __launch_bounds__(NT) void solve_bwd_kernel(DevState S,
+// Dynamic LDS: t (tcap >= m doubles), the packed pivot block, NTH doubles.
+template <int NTH>
+__global__ __launch_bounds__(NTH) void solve_bwd_kernel(DevState S,
                                                        const int* __restrict__ lnodes,
                                                        double* __restrict__ x,
                                                        int ldx, int nrhs, int tcap) {
@@ -2403,15 +2457,15 @@ __global__ __launch_bounds__(NT) void solve_bwd_kernel(DevState S,
   const int* vars = S.ipool + S.voff[node];
   const int ld = m, tid = threadIdx.x;
   const bool wave = p <= 64;
-  if (wave) stage_pivot_block(L, ld, p, Ls);
+  if (wave) stage_pivot_block<NTH>(L, ld, p, Ls);
   for (int r = 0; r < nrhs; r++) {
     double* xr = x + (size_t)r * ldx;
-    for (int j = tid; j < m; j += NT) {
+    for (int j = tid; j < m; j += NTH) {
       int v = vars[j];
       t[j] = xr[v >= 0 ? v : -v - 1];
     }
     __syncthreads();
-    front_update_t(L, ld, p, m, t);
+    front_update_t<NTH>(L, ld, p, m, t, Ls + PIVOT_TRI);
     if (wave) {
       if (tid < 64) {
         const double tl = wave_upper_solve(tid < p ? t[tid] : 0.0, tid, p, vars, Ls);
@@ -2426,17 +2480,17 @@ __global__ __launch_bounds__(NT) void solve_bwd_kernel(DevState S,
     for (int i = p - 1; i >= 0;) {
       if (vars[i] >= 0) {
         const double ti = t[i];
-        for (int q = tid; q < i; q += NT) t[q] -= L[i + (size_t)q * ld] * ti;
+        for (int q = tid; q < i; q += NTH) t[q] -= L[i + (size_t)q * ld] * ti;
         i -= 1;
       } else {
         const double t2 = t[i], t1 = t[i - 1];
-        for (int q = tid; q < i - 1; q += NT)
+        for (int q = tid; q < i - 1; q += NTH)
           t[q] -= L[i + (size_t)q * ld] * t2 + L[i - 1 + (size_t)q * ld] * t1;
         i -= 2;
       }
       __syncthreads();
     }
-    for (int j = tid; j < p; j += NT) {
+    for (int j = tid; j < p; j += NTH) {
       int v = vars[j];
       xr[v >= 0 ? v : -v - 1] = t[j];
     }
@@ -2678,6 +2732,7 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   std::vector<int64_t> big_woff;
   int64_t wmax = 0;
   big_nth_.assign(nlev, NTP);
+  solve_wide_.assign(nlev, 0);
   for (int l = 0; l < nlev; l++) {
     int mx = 0;
     int big_mcap = 0;
@@ -2701,6 +2756,13 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
     }
     level_lds_[l] = mx * mx;
     big_nth_[l] = big_mcap <= NTP / 4 ? NTP / 4 : big_mcap <= NTP / 2 ? NTP / 2 : NTP;
+    // Solves: levels of few tall fronts take 1,024-thread workgroups (a
+    // front's update reads then keep four times the loads in flight).
+    int smax = 0;
+    for (int t = level_ptr_[l]; t < level_ptr_[l + 1]; t++)
+      smax = std::max(smax, ns[level_nodes_[t]] + cs[level_nodes_[t]]);
+    solve_wide_[l] = (level_ptr_[l + 1] - level_ptr_[l] <= SOLVE_WIDE_FRONTS &&
+                      smax >= SOLVE_WIDE_ROWS) ? 1 : 0;
     big_ptr_[l + 1] = int(big_nodes_.size());
     wmax = std::max(wmax, wtot);
   }
@@ -2791,10 +2853,16 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
       HIPCHK(hipFuncSetAttribute((const void*)big_assemble_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  150 * 1024));
-      HIPCHK(hipFuncSetAttribute((const void*)solve_fwd_kernel,
+      HIPCHK(hipFuncSetAttribute((const void*)solve_fwd_kernel<NT>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  150 * 1024));
-      HIPCHK(hipFuncSetAttribute((const void*)solve_bwd_kernel,
+      HIPCHK(hipFuncSetAttribute((const void*)solve_bwd_kernel<NT>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 150 * 1024));
+      HIPCHK(hipFuncSetAttribute((const void*)solve_fwd_kernel<NTS>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 150 * 1024));
+      HIPCHK(hipFuncSetAttribute((const void*)solve_bwd_kernel<NTS>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  150 * 1024));
       attr_done = true;
@@ -3187,7 +3255,8 @@ int LdlNumeric::enqueue_solve(int nrhs, double* x, int ldx, hipStream_t stream,
                               std::string& err) {
   const int tcap = max_p_;
   const size_t lds = size_t(tcap + PIVOT_TRI + NT) * sizeof(double);
-  if (lds > 150 * 1024) {
+  const size_t lds_wide = size_t(tcap + PIVOT_TRI + NTS) * sizeof(double);
+  if (lds_wide > 150 * 1024) {
     err = "front too large for the LDS solve kernels";
     return -1;
   }
@@ -3204,16 +3273,24 @@ int LdlNumeric::enqueue_solve(int nrhs, double* x, int ldx, hipStream_t stream,
     const int cnt = LP[l + 1] - LP[l];
     if (cnt == 0) continue;
     tk_begin(stream);
-    hipLaunchKernelGGL(solve_fwd_kernel, dim3(cnt), dim3(NT), lds, stream, P_, S_,
-                       LN + LP[l], x, ldx, nrhs, ub_, (size_t)S_.icap, tcap);
+    if (solve_wide_[l])
+      hipLaunchKernelGGL(solve_fwd_kernel<NTS>, dim3(cnt), dim3(NTS), lds_wide, stream, P_, S_,
+                         LN + LP[l], x, ldx, nrhs, ub_, (size_t)S_.icap, tcap);
+    else
+      hipLaunchKernelGGL(solve_fwd_kernel<NT>, dim3(cnt), dim3(NT), lds, stream, P_, S_,
+                         LN + LP[l], x, ldx, nrhs, ub_, (size_t)S_.icap, tcap);
     tk_end(stream, TK_SOLVE_FWD);
   }
   for (int l = nlev - 1; l >= 0; l--) {
     const int cnt = LP[l + 1] - LP[l];
     if (cnt == 0) continue;
     tk_begin(stream);
-    hipLaunchKernelGGL(solve_bwd_kernel, dim3(cnt), dim3(NT), lds, stream, S_,
-                       LN + LP[l], x, ldx, nrhs, tcap);
+    if (solve_wide_[l])
+      hipLaunchKernelGGL(solve_bwd_kernel<NTS>, dim3(cnt), dim3(NTS), lds_wide, stream, S_,
+                         LN + LP[l], x, ldx, nrhs, tcap);
+    else
+      hipLaunchKernelGGL(solve_bwd_kernel<NT>, dim3(cnt), dim3(NT), lds, stream, S_,
+                         LN + LP[l], x, ldx, nrhs, tcap);
     tk_end(stream, TK_SOLVE_BWD);
   }
   if (scaling_)
