@@ -1179,19 +1179,54 @@ struct PanelShared {
   int cnt[4];
 };
 
-// s - sum_{q < kw} Ls[q][t] w[q], kw <= LS_COLS.  Fully unrolled with the
-// bound as a predicate, so all LDS reads are issued before the first FMA
-// waits on one.
+// s - sum_{q < kw} Ls[q][t] w[q] over NC chunks of 4 columns (4 (NC - 1)
+// < kw <= 4 NC): every LDS read is issued before the first FMA waits on
+// one; in the last chunk the columns past kw count as zero (their LDS
+// entries are read, in bounds, and dropped).
+template <int NTH, int NC>
+__device__ __forceinline__ double lds_dot_n(double s, const double* Ls, int t, const double* w,
+                                            int kw) {
+  double a[4 * NC], b[4 * NC];
+#pragma unroll
+  for (int q = 0; q < 4 * NC; q++) {
+    a[q] = Ls[q * NTH + t];
+    b[q] = w[q];
+  }
+  double s1 = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4 * NC; q++) {
+    double aq = a[q], bq = b[q];
+    if (q >= 4 * (NC - 1)) {
+      aq = q < kw ? aq : 0.0;
+      bq = q < kw ? bq : 0.0;
+    }
+    if (q & 1)
+      s1 -= aq * bq;
+    else
+      s -= aq * bq;
+  }
+  return s + s1;
+}
+
+// s - sum_{q < kw} Ls[q][t] w[q] for a workgroup-uniform kw < NBR: one
+// uniform branch on the chunk count, so the loads are not serialized
+// behind per-column branches.
 template <int NTH>
 __device__ __forceinline__ double lds_dot(double s, const double* Ls, int t, const double* w,
                                           int kw) {
-  double s1 = 0.0;
-#pragma unroll
-  for (int q = 0; q < LS_COLS; q += 2) {
-    if (q < kw) s -= Ls[q * NTH + t] * w[q];
-    if (q + 1 < kw) s1 -= Ls[(q + 1) * NTH + t] * w[q + 1];
+  static_assert(NBR <= 16 && LS_COLS >= 16, "four chunks of four panel columns");
+  switch ((kw + 3) >> 2) {
+    case 0:
+      return s;
+    case 1:
+      return lds_dot_n<NTH, 1>(s, Ls, t, w, kw);
+    case 2:
+      return lds_dot_n<NTH, 2>(s, Ls, t, w, kw);
+    case 3:
+      return lds_dot_n<NTH, 3>(s, Ls, t, w, kw);
+    default:
+      return lds_dot_n<NTH, 4>(s, Ls, t, w, kw);
   }
-  return s + s1;
 }
 
 // Exchange the current column values s, s2 of the owners of rows a, b
