@@ -474,6 +474,53 @@ __device__ __forceinline__ double lane_d(double v, int l) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// Maxima of non-negative doubles (-1 for "none"; no NaN) as maxima of the
+// keys bits(x) + 1 (0 for none, +0 and -0 alike): for non-negative values
+// the unsigned order of the bit patterns is the order of the values.  A
+// 64-bit key maximum is taken on the high words with 32-bit DPP maxima and
+// then, only when several lanes share the largest high word, on their low
+// words.
+__device__ __forceinline__ unsigned long long fmax_key(double x) {
+  return x > 0.0 ? (unsigned long long)__double_as_longlong(x) + 1ull : (x == 0.0 ? 1ull : 0ull);
+}
+__device__ __forceinline__ double fmax_key_value(unsigned long long k) {
+  return k ? __longlong_as_double((long long)(k - 1ull)) : -1.0;
+}
+// Maximum of x over the wave (ROW = false) or over each row of 16 lanes
+// (ROW = true), left in lane 63 (the row's lane 15): lanes without a source
+// take the identity 0.
+template <bool ROW>
+__device__ __forceinline__ unsigned dpp_max_u32(unsigned x) {
+#define AMIGO_DPP_UMAX(ctrl, rm, bm) \
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, ctrl, rm, bm, false));
+  AMIGO_DPP_UMAX(0x111, 0xf, 0xf)  // row_shr:1
+  AMIGO_DPP_UMAX(0x112, 0xf, 0xf)  // row_shr:2
+  AMIGO_DPP_UMAX(0x113, 0xf, 0xf)  // row_shr:3
+  AMIGO_DPP_UMAX(0x114, 0xf, 0xe)  // row_shr:4
+  AMIGO_DPP_UMAX(0x118, 0xf, 0xc)  // row_shr:8
+  if (!ROW) {
+    AMIGO_DPP_UMAX(0x142, 0xa, 0xf)  // row_bcast:15
+    AMIGO_DPP_UMAX(0x143, 0xc, 0xf)  // row_bcast:31
+  }
+#undef AMIGO_DPP_UMAX
+  return x;
+}
+// Largest key of the wave (lanes 0..15 only with ROW), in every lane.  All
+// lanes active.
+template <bool ROW = false>
+__device__ __forceinline__ unsigned long long wave_max_key64(unsigned long long key) {
+  constexpr int last = ROW ? 15 : 63;
+  const unsigned hi = (unsigned)(key >> 32), lo = (unsigned)key;
+  if (__ballot(key != 0ull) == 0ull) return 0ull;
+  const unsigned hm = (unsigned)__builtin_amdgcn_readlane((int)dpp_max_u32<ROW>(hi), last);
+  const unsigned long long at = __ballot(hi == hm) & (ROW ? 0xffffull : ~0ull);
+  unsigned lm;
+  if ((at & (at - 1ull)) == 0ull)
+    lm = (unsigned)__builtin_amdgcn_readlane((int)lo, __ffsll((long long)at) - 1);
+  else
+    lm = (unsigned)__builtin_amdgcn_readlane((int)dpp_max_u32<ROW>(hi == hm ? lo : 0u), last);
+  return ((unsigned long long)hm << 32) | lm;
+}
 // One DPP step of a wave maximum of doubles: lanes without a source keep
 // their own value.
 #define AMIGO_DPP_FMAX(x, ctrl, rm, bm)                                                        \
@@ -484,7 +531,10 @@ __device__ __forceinline__ double lane_d(double v, int l) {
                                                 false);                                        \
     x = fmax(x, __longlong_as_double(((long long)hi_ << 32) | (unsigned)lo_));                 \
   }
-// Wave maximum of non-negative values (-1 for "none"; no NaN), every lane.
+// Wave maximum of non-negative values (-1 for "none"; no NaN), every lane:
+// a branch-free chain of DPP steps (the LDS fronts' pivot search, where a
+// single wave's latency counts).  The panel's block maximum, where the
+// VALU work of all waves counts, uses the key form above.
 __device__ __forceinline__ double wave_fmax(double x) {
   AMIGO_DPP_FMAX(x, 0x111, 0xf, 0xf)  // row_shr:1
   AMIGO_DPP_FMAX(x, 0x112, 0xf, 0xf)  // row_shr:2
@@ -1171,7 +1221,7 @@ constexpr size_t PANEL_LDS_BYTES = sizeof(double) * LS_COLS * NTP;
 
 struct PanelShared {
   VI red[2 * NTP / 64];
-  double wmax[2][NTP / 64];  // per-wave column maxima (double-buffered)
+  unsigned long long wkey[2][NTP / 64];  // per-wave column maxima as keys (double-buffered)
   double wrow[2][NBR + 2];   // row k of W (double-buffered)
   double wrow2[NBR + 2];
   double xch[4];
@@ -1179,54 +1229,61 @@ struct PanelShared {
   int cnt[4];
 };
 
-// s - sum_{q < kw} Ls[q][t] w[q] over NC chunks of 4 columns (4 (NC - 1)
-// < kw <= 4 NC): every LDS read is issued before the first FMA waits on
-// one; in the last chunk the columns past kw count as zero (their LDS
-// entries are read, in bounds, and dropped).
-template <int NTH, int NC>
-__device__ __forceinline__ double lds_dot_n(double s, const double* Ls, int t, const double* w,
-                                            int kw) {
-  double a[4 * NC], b[4 * NC];
+// Columns Q0 .. Q0 + NQ - 1 of s - sum_{q < kw} Ls[q][t] w[q] (even columns
+// into s, odd into s1): all NQ LDS reads are issued before the first FMA
+// waits on one.  In the last part (LAST) the columns past kw count as zero
+// (their LDS entries are read, in bounds, and dropped).
+template <int NTH, int Q0, int NQ, bool LAST>
+__device__ __forceinline__ void lds_dot_part(double& s, double& s1, const double* Ls, int t,
+                                             const double* w, int kw) {
+  double a[NQ], b[NQ];
 #pragma unroll
-  for (int q = 0; q < 4 * NC; q++) {
-    a[q] = Ls[q * NTH + t];
-    b[q] = w[q];
+  for (int q = 0; q < NQ; q++) {
+    a[q] = Ls[(Q0 + q) * NTH + t];
+    b[q] = w[Q0 + q];
   }
-  double s1 = 0.0;
 #pragma unroll
-  for (int q = 0; q < 4 * NC; q++) {
+  for (int q = 0; q < NQ; q++) {
     double aq = a[q], bq = b[q];
-    if (q >= 4 * (NC - 1)) {
-      aq = q < kw ? aq : 0.0;
-      bq = q < kw ? bq : 0.0;
+    if (LAST && q >= NQ - 4) {
+      aq = Q0 + q < kw ? aq : 0.0;
+      bq = Q0 + q < kw ? bq : 0.0;
     }
-    if (q & 1)
+    if ((Q0 + q) & 1)
       s1 -= aq * bq;
     else
       s -= aq * bq;
   }
-  return s + s1;
 }
 
 // s - sum_{q < kw} Ls[q][t] w[q] for a workgroup-uniform kw < NBR: one
-// uniform branch on the chunk count, so the loads are not serialized
-// behind per-column branches.
+// uniform branch on the number of 4-column chunks, then at most two
+// batches of reads (8 columns each), so that the loads are not serialized
+// behind per-column branches and the registers in flight stay bounded.
 template <int NTH>
 __device__ __forceinline__ double lds_dot(double s, const double* Ls, int t, const double* w,
                                           int kw) {
   static_assert(NBR <= 16 && LS_COLS >= 16, "four chunks of four panel columns");
+  double s1 = 0.0;
   switch ((kw + 3) >> 2) {
     case 0:
       return s;
     case 1:
-      return lds_dot_n<NTH, 1>(s, Ls, t, w, kw);
+      lds_dot_part<NTH, 0, 4, true>(s, s1, Ls, t, w, kw);
+      break;
     case 2:
-      return lds_dot_n<NTH, 2>(s, Ls, t, w, kw);
+      lds_dot_part<NTH, 0, 8, true>(s, s1, Ls, t, w, kw);
+      break;
     case 3:
-      return lds_dot_n<NTH, 3>(s, Ls, t, w, kw);
+      lds_dot_part<NTH, 0, 8, false>(s, s1, Ls, t, w, kw);
+      lds_dot_part<NTH, 8, 4, true>(s, s1, Ls, t, w, kw);
+      break;
     default:
-      return lds_dot_n<NTH, 4>(s, Ls, t, w, kw);
+      lds_dot_part<NTH, 0, 8, false>(s, s1, Ls, t, w, kw);
+      lds_dot_part<NTH, 8, 8, true>(s, s1, Ls, t, w, kw);
+      break;
   }
+  return s + s1;
 }
 
 // Exchange the current column values s, s2 of the owners of rows a, b
@@ -1258,14 +1315,13 @@ __device__ __forceinline__ void swap_lds_rows_values(double& s, double& s2, int 
 template <int NTH>
 __device__ __forceinline__ double block_fmax(double x, PanelShared& sh, int buf) {
   constexpr int NW = NTH / 64;
+  static_assert(NW <= 16, "one wave maximum per lane of a DPP row");
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const double w = wave_fmax(x);
-  if (lane == 0) sh.wmax[buf][wid] = w;
+  const unsigned long long w = wave_max_key64(fmax_key(x));
+  if (lane == 0) sh.wkey[buf][wid] = w;
   __syncthreads();
-  double r = sh.wmax[buf][0];
-#pragma unroll
-  for (int q = 1; q < NW; q++) r = fmax(r, sh.wmax[buf][q]);
-  return r;
+  // The waves' maxima in lanes 0..NW-1 of every wave, one row maximum.
+  return fmax_key_value(wave_max_key64<true>(lane < NW ? sh.wkey[buf][lane] : 0ull));
 }
 
 // Panel block <-> global front: columns [kstart, kstart + pw), rows >= the
@@ -1415,6 +1471,7 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
     int step = 1, kpiv = k;
     bool zero = false, plain = true;
     double colmax0 = 0.0, akk = 0.0;
+    double wn2 = 0.0;  // W[k + 2, wlane]: the next W row after a 2x2 pivot at k, k + 1
     if (chol) {
       __syncthreads();
     } else {
@@ -1438,6 +1495,8 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
         const double colmax = colmax0;
         if (has_cand) {
           if (wlane >= 0 && wlane < kw) sh.wrow2[wlane] = W[imax + (size_t)wlane * ldw];
+          if (imax == k + 1 && wlane >= 0 && wlane < kw && k + 2 < m)
+            wn2 = W[k + 2 + (size_t)wlane * ldw];
           // Not yet updated column imax, rows >= k: rows below imax from its
           // column, rows k..imax-1 from row imax (the columns r < imax).
           double fi = 0.0;
@@ -1514,6 +1573,17 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
       continue;
     }
     const int ksrc = k + step - 1;
+    // A 2x2 pivot on rows k, k + 1 (no interchange): the next W row, row
+    // k + 2, is published now (its earlier entries read above, this step's
+    // two entries its own values), so the step ends with one barrier.
+    const bool next2 = step == 2 && kpiv == k + 1;
+    if (next2) {
+      if (wlane >= 0 && wlane < kw && k + 2 < m) sh.wrow[wb ^ 1][wlane] = wn2;
+      if (own && r == k + 2) {
+        sh.wrow[wb ^ 1][kw] = s;
+        sh.wrow[wb ^ 1][kw + 1] = s2;
+      }
+    }
     if (kpiv != ksrc) {
       __syncthreads();
       interchange(ksrc, kpiv, kw + step, s, s2);
@@ -1552,7 +1622,9 @@ __device__ void panel_lds(double* F, double* W, int* vars, int m, int& k, int& n
     k += step;
     kw += step;
     wb ^= 1;
-    if (!plain || step != 1) {
+    if (next2) {
+      __syncthreads();
+    } else if (!plain || step != 1) {
       // The next pivot row's W row was not published speculatively.
       __syncthreads();
       publish(wb);
