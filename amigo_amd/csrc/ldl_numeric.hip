@@ -370,108 +370,12 @@ __device__ int factor_unblocked(double* F, int ld, int m, int nfs, int* vars,
   return k;
 }
 
-// Column statistics of column k in one pass: the largest candidate entry
-// (rows k+1..nc-1, with its row) and the largest contribution entry (rows
-// nc..m-1).  The column is cached in Wv on the way.  Two barriers.
 // Order-preserving key of a candidate |value| for max reductions: 0 for
 // "no candidate" (and NaN, which never wins, as in vi_best), otherwise the
 // bit pattern of the non-negative double plus one.
 __device__ __forceinline__ unsigned long long cand_key(VI c) {
   if (!(c.v >= 0.0)) return 0ull;
   return (unsigned long long)__double_as_longlong(c.v) + 1ull;
-}
-
-// Wave-wide maximum of a 64-bit key with DPP row shifts and row broadcasts
-// (the result is in lane 63, read back to every lane).
-#define AMIGO_DPP_MAX_STEP(x, ctrl, rm, bm)                                                  \
-  {                                                                                          \
-    const unsigned lo_ = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(x), ctrl, \
-                                                               rm, bm, false);               \
-    const unsigned hi_ = (unsigned)__builtin_amdgcn_update_dpp(                              \
-        0, (int)(unsigned)((x) >> 32), ctrl, rm, bm, false);                                 \
-    const unsigned long long y_ = ((unsigned long long)hi_ << 32) | lo_;                     \
-    x = y_ > x ? y_ : x;                                                                     \
-  }
-__device__ __forceinline__ unsigned long long wave_max_key(unsigned long long x) {
-  AMIGO_DPP_MAX_STEP(x, 0x111, 0xf, 0xf)  // row_shr:1
-  AMIGO_DPP_MAX_STEP(x, 0x112, 0xf, 0xf)  // row_shr:2
-  AMIGO_DPP_MAX_STEP(x, 0x113, 0xf, 0xf)  // row_shr:3
-  AMIGO_DPP_MAX_STEP(x, 0x114, 0xf, 0xe)  // row_shr:4
-  AMIGO_DPP_MAX_STEP(x, 0x118, 0xf, 0xc)  // row_shr:8
-  AMIGO_DPP_MAX_STEP(x, 0x142, 0xa, 0xf)  // row_bcast:15
-  AMIGO_DPP_MAX_STEP(x, 0x143, 0xc, 0xf)  // row_bcast:31
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, 63);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), 63);
-  return ((unsigned long long)hi << 32) | lo;
-}
-// Maximum over each row of 16 lanes, left in the row's last lane.
-__device__ __forceinline__ unsigned long long row_max_key(unsigned long long x) {
-  AMIGO_DPP_MAX_STEP(x, 0x111, 0xf, 0xf)  // row_shr:1
-  AMIGO_DPP_MAX_STEP(x, 0x112, 0xf, 0xf)  // row_shr:2
-  AMIGO_DPP_MAX_STEP(x, 0x113, 0xf, 0xf)  // row_shr:3
-  AMIGO_DPP_MAX_STEP(x, 0x114, 0xf, 0xe)  // row_shr:4
-  AMIGO_DPP_MAX_STEP(x, 0x118, 0xf, 0xc)  // row_shr:8
-  return x;
-}
-#undef AMIGO_DPP_MAX_STEP
-
-__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long x, int l) {
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, l);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), l);
-  return ((unsigned long long)hi << 32) | lo;
-}
-
-// Two block-wide argmax reductions (largest value, smallest index on ties;
-// lanes hold at most one candidate each, and a lane's index grows with the
-// lane id inside a wave) -> all threads.  red holds 2 * NTH / 64 entries.
-// The caller guarantees a barrier between consecutive calls.
-template <int NTH>
-__device__ void block_argmax2_dpp(VI& x, VI& y, VI* red) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const unsigned long long kx = cand_key(x), ky = cand_key(y);
-  const unsigned long long mx = wave_max_key(kx), my = wave_max_key(ky);
-  const unsigned long long bx = __ballot(kx == mx && mx != 0ull);
-  const unsigned long long by = __ballot(ky == my && my != 0ull);
-  const int ix = bx ? __builtin_amdgcn_readlane(x.i, __ffsll((long long)bx) - 1) : INT_MAX;
-  const int iy = by ? __builtin_amdgcn_readlane(y.i, __ffsll((long long)by) - 1) : INT_MAX;
-  if (lane == 0) {
-    red[wid] = VI{mx ? __longlong_as_double((long long)(mx - 1ull)) : -1.0, ix};
-    red[NTH / 64 + wid] = VI{my ? __longlong_as_double((long long)(my - 1ull)) : -1.0, iy};
-  }
-  __syncthreads();
-  // Across the waves, in every wave: the waves' x winners in lanes 0..NW-1
-  // (row 0), their y winners in lanes 32..32+NW-1 (row 2), one DPP row
-  // maximum each; equal keys go to the lowest lane, i.e. the lowest wave,
-  // whose rows come first.
-  constexpr int NW = NTH / 64;
-  static_assert(NW <= 16, "one wave winner per lane of a DPP row");
-  VI c = {-1.0, INT_MAX};
-  if (lane < NW)
-    c = red[lane];
-  else if (lane >= 32 && lane < 32 + NW)
-    c = red[NW + lane - 32];
-  const unsigned long long ck = cand_key(c);
-  const unsigned long long rk = row_max_key(ck);
-  const unsigned long long kx2 = readlane_u64(rk, 15), ky2 = readlane_u64(rk, 47);
-  const unsigned long long wx = __ballot(ck == kx2 && kx2 != 0ull) & 0xffffull;
-  const unsigned long long wy = __ballot(ck == ky2 && ky2 != 0ull) & (0xffffull << 32);
-  x.v = kx2 ? __longlong_as_double((long long)(kx2 - 1ull)) : -1.0;
-  x.i = wx ? __builtin_amdgcn_readlane(c.i, __ffsll((long long)wx) - 1) : INT_MAX;
-  y.v = ky2 ? __longlong_as_double((long long)(ky2 - 1ull)) : -1.0;
-  y.i = wy ? __builtin_amdgcn_readlane(c.i, __ffsll((long long)wy) - 1) : INT_MAX;
-}
-
-// ---- wave-local pivot search of an LDS-resident front ---------------------
-// Every wave reads the column it needs from LDS (two rows per lane: fronts
-// of at most LDS_MAX_M = 128 rows) and reduces it with DPP, so all waves
-// hold the same pivot statistics without exchanging them through LDS.
-static_assert(LDS_MAX_M <= 128, "two rows per lane in the wave-local pivot search");
-
-__device__ __forceinline__ double lane_d(double v, int l) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)b, l);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 // Maxima of non-negative doubles (-1 for "none"; no NaN) as maxima of the
@@ -521,6 +425,58 @@ __device__ __forceinline__ unsigned long long wave_max_key64(unsigned long long 
     lm = (unsigned)__builtin_amdgcn_readlane((int)dpp_max_u32<ROW>(hi == hm ? lo : 0u), last);
   return ((unsigned long long)hm << 32) | lm;
 }
+// Two block-wide argmax reductions (largest value, smallest index on ties;
+// lanes hold at most one candidate each, and a lane's index grows with the
+// lane id inside a wave) -> all threads.  red holds 2 * NTH / 64 entries.
+// The caller guarantees a barrier between consecutive calls.
+template <int NTH>
+__device__ void block_argmax2_dpp(VI& x, VI& y, VI* red) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const unsigned long long kx = cand_key(x), ky = cand_key(y);
+  const unsigned long long mx = wave_max_key64(kx), my = wave_max_key64(ky);
+  const unsigned long long bx = __ballot(kx == mx && mx != 0ull);
+  const unsigned long long by = __ballot(ky == my && my != 0ull);
+  const int ix = bx ? __builtin_amdgcn_readlane(x.i, __ffsll((long long)bx) - 1) : INT_MAX;
+  const int iy = by ? __builtin_amdgcn_readlane(y.i, __ffsll((long long)by) - 1) : INT_MAX;
+  if (lane == 0) {
+    red[wid] = VI{mx ? __longlong_as_double((long long)(mx - 1ull)) : -1.0, ix};
+    red[NTH / 64 + wid] = VI{my ? __longlong_as_double((long long)(my - 1ull)) : -1.0, iy};
+  }
+  __syncthreads();
+  // Across the waves, in every wave: the waves' x winners in lanes 0..NW-1
+  // (row 0), their y winners in lanes 32..32+NW-1 (row 2), one DPP row
+  // maximum each; equal keys go to the lowest lane, i.e. the lowest wave,
+  // whose rows come first.
+  constexpr int NW = NTH / 64;
+  static_assert(NW <= 16, "one wave winner per lane of a DPP row");
+  VI cx = {-1.0, INT_MAX}, cy = {-1.0, INT_MAX};
+  if (lane < NW) {
+    cx = red[lane];
+    cy = red[NW + lane];
+  }
+  const unsigned long long ckx = cand_key(cx), cky = cand_key(cy);
+  const unsigned long long kx2 = wave_max_key64<true>(ckx), ky2 = wave_max_key64<true>(cky);
+  const unsigned long long wx = __ballot(ckx == kx2 && kx2 != 0ull) & 0xffffull;
+  const unsigned long long wy = __ballot(cky == ky2 && ky2 != 0ull) & 0xffffull;
+  x.v = kx2 ? __longlong_as_double((long long)(kx2 - 1ull)) : -1.0;
+  x.i = wx ? __builtin_amdgcn_readlane(cx.i, __ffsll((long long)wx) - 1) : INT_MAX;
+  y.v = ky2 ? __longlong_as_double((long long)(ky2 - 1ull)) : -1.0;
+  y.i = wy ? __builtin_amdgcn_readlane(cy.i, __ffsll((long long)wy) - 1) : INT_MAX;
+}
+
+// ---- wave-local pivot search of an LDS-resident front ---------------------
+// Every wave reads the column it needs from LDS (two rows per lane: fronts
+// of at most LDS_MAX_M = 128 rows) and reduces it with DPP, so all waves
+// hold the same pivot statistics without exchanging them through LDS.
+static_assert(LDS_MAX_M <= 128, "two rows per lane in the wave-local pivot search");
+
+__device__ __forceinline__ double lane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // One DPP step of a wave maximum of doubles: lanes without a source keep
 // their own value.
 #define AMIGO_DPP_FMAX(x, ctrl, rm, bm)                                                        \
