@@ -2115,8 +2115,9 @@ constexpr int SOLVE_WIDE_ROWS = 256;
 __device__ __forceinline__ int tri_off(int j, int p) { return j * p - ((j * (j - 1)) >> 1); }
 
 // All threads; 16 loads per thread are in flight before any is stored.
-// Ends with a barrier.
-template <int NTH>
+// Ends with a barrier when SYNC (otherwise the caller's next barrier
+// publishes the block).
+template <int NTH, bool SYNC = true>
 __device__ __forceinline__ void stage_pivot_block(const double* __restrict__ L, int ld, int p,
                                                   double* Ls) {
   const int tot = p * p;
@@ -2133,7 +2134,7 @@ __device__ __forceinline__ void stage_pivot_block(const double* __restrict__ L, 
       if (e < tot && i >= j) Ls[tri_off(j, p) + i - j] = a[u];
     }
   }
-  __syncthreads();
+  if (SYNC) __syncthreads();
 }
 
 // Forward: unit-lower solve of t[0:p) (in place, lane l <- t[l]) and the
@@ -2398,7 +2399,21 @@ __global__ __launch_bounds__(NTH) void solve_fwd_kernel(DevPlan P, DevState S,
   const int kb = P.kptr[node], ke = P.kptr[node + 1];
   const int* fpos = S.fpos + S.voff[node];
   const bool wave = p <= 64;
-  if (wave) stage_pivot_block<NTH>(L, ld, p, Ls);
+  // The pivot block, the children's metadata (when one batch holds them
+  // all) and the right-hand side are read together, before one barrier.
+  const bool one_batch = ke - kb <= NT;
+  auto load_kmeta = [&](int q0, int nb) {
+    if (tid < nb) {
+      const int c = P.kids[q0 + tid];
+      const int pc = S.npiv[c];
+      kmeta[0][tid] = S.ndel[c];
+      kmeta[1][tid] = S.m[c] - pc;
+      kmeta[2][tid] = S.voff[c] + pc;
+      kmeta[3][tid] = P.cptr[c];
+    }
+  };
+  if (wave) stage_pivot_block<NTH, false>(L, ld, p, Ls);
+  if (one_batch) load_kmeta(kb, ke - kb);
   for (int r = 0; r < nrhs; r++) {
     double* xr = x + (size_t)r * ldx;
     double* ur = ub + (size_t)r * ubld;
@@ -2414,15 +2429,10 @@ __global__ __launch_bounds__(NTH) void solve_fwd_kernel(DevPlan P, DevState S,
     int doff = ns;
     for (int q0 = kb; q0 < ke; q0 += NT) {  // children in batches of NT
       const int nb = min(NT, ke - q0);
-      if (tid < nb) {
-        const int c = P.kids[q0 + tid];
-        const int pc = S.npiv[c];
-        kmeta[0][tid] = S.ndel[c];
-        kmeta[1][tid] = S.m[c] - pc;
-        kmeta[2][tid] = S.voff[c] + pc;
-        kmeta[3][tid] = P.cptr[c];
+      if (!one_batch) {
+        load_kmeta(q0, nb);
+        __syncthreads();
       }
-      __syncthreads();
       for (int qq = 0; qq < nb; qq++) {
         const int dc = kmeta[0][qq], wc = kmeta[1][qq];
         const double* uc = ur + kmeta[2][qq];
@@ -2520,7 +2530,8 @@ __global__ __launch_bounds__(NTH) void solve_bwd_kernel(DevState S,
   const int* vars = S.ipool + S.voff[node];
   const int ld = m, tid = threadIdx.x;
   const bool wave = p <= 64;
-  if (wave) stage_pivot_block<NTH>(L, ld, p, Ls);
+  // The pivot block and the right-hand side are read before one barrier.
+  if (wave) stage_pivot_block<NTH, false>(L, ld, p, Ls);
   for (int r = 0; r < nrhs; r++) {
     double* xr = x + (size_t)r * ldx;
     for (int j = tid; j < m; j += NTH) {
