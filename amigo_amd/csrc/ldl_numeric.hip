@@ -2107,8 +2107,9 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 // Pivot blocks L[0:p, 0:p] (p <= 64) are staged in LDS as packed lower
 // triangles: column j starts at tri_off(j, p).
 constexpr int PIVOT_TRI = 64 * 65 / 2;
-// Solve workgroups: NT threads, or NTS on levels of at most SOLVE_WIDE_FRONTS
-// fronts with one of at least SOLVE_WIDE_ROWS rows.
+// Forward-solve workgroups: NT threads, or NTS on levels of at most
+// SOLVE_WIDE_FRONTS fronts with one of at least SOLVE_WIDE_ROWS rows (the
+// arithmetic of a front does not depend on the width).
 constexpr int NTS = 1024;
 constexpr int SOLVE_WIDE_FRONTS = 256;
 constexpr int SOLVE_WIDE_ROWS = 256;
@@ -2209,23 +2210,27 @@ template <int NTH>
 __device__ __forceinline__ void front_update(const double* __restrict__ L, int ld, int p, int m,
                                              const double* t, double* __restrict__ uo,
                                              double* red) {
+  constexpr int CH = 16;  // loads in flight per thread
   const int tid = threadIdx.x, R = m - p;
   if (R <= 0) return;
   if (R < NTH) {
-    const int G = max(1, min(NTH / R, p));
+    // G from NT, not NTH: a row's partial sums (and so its rounding) do not
+    // depend on the workgroup size, so a front solves to the same bits
+    // whichever width its level is launched with (alone or in a batch).
+    const int G = max(1, min(NT / R, p));
     double s = 0.0;
     if (tid < G * R) {
       const int g = tid / R, i = p + tid - g * R;
       const double* Li = L + i;
-      for (int j0 = g; j0 < p; j0 += 16 * G) {
-        double a[16];
+      for (int j0 = g; j0 < p; j0 += CH * G) {
+        double a[CH];
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
+        for (int u = 0; u < CH; u++) {
           const int j = j0 + u * G;
           a[u] = j < p ? Li[(size_t)j * ld] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
+        for (int u = 0; u < CH; u++) {
           const int j = j0 + u * G;
           if (j < p) s += a[u] * t[j];
         }
@@ -2242,12 +2247,12 @@ __device__ __forceinline__ void front_update(const double* __restrict__ L, int l
     for (int i = p + tid; i < m; i += NTH) {
       const double* Li = L + i;
       double s = 0.0;
-      for (int j0 = 0; j0 < p; j0 += 16) {
-        double a[16];
+      for (int j0 = 0; j0 < p; j0 += CH) {
+        double a[CH];
 #pragma unroll
-        for (int u = 0; u < 16; u++) a[u] = j0 + u < p ? Li[(size_t)(j0 + u) * ld] : 0.0;
+        for (int u = 0; u < CH; u++) a[u] = j0 + u < p ? Li[(size_t)(j0 + u) * ld] : 0.0;
 #pragma unroll
-        for (int u = 0; u < 16; u++)
+        for (int u = 0; u < CH; u++)
           if (j0 + u < p) s += a[u] * t[j0 + u];
       }
       uo[i] = t[i] - s;
@@ -2283,60 +2288,16 @@ __device__ __forceinline__ double wave_reduce16(double (&v)[16], int lane) {
 }
 
 // Front update of the backward solve: t[j] -= sum_{i in [p,m)} L[i, j] t[i]
-// for j < p.  With at least as many 16-column blocks as waves, wave w takes
-// columns jb + w + NW u (u < 16) of each block of 16 NW columns and its
-// lanes run over all the rows 64 at a time.  With more waves than column
-// blocks (few pivots, a 1024-thread workgroup), the RS = NW / CB waves of a
-// block split its rows (wave rs of the block takes row groups rs, rs + RS,
-// ...) and their sums are added in rs order through red (NW * 16 doubles of
-// LDS).  16 loads in flight per lane; deterministic.  All threads; ends
-// with a barrier.
+// for j < p.  Wave w takes columns jb + w + NW u (u < 16) of each block of
+// 16 NW columns and its lanes run over all the rows 64 at a time, with 16
+// loads in flight.  All threads; ends with a barrier.
 template <int NTH>
 __device__ __forceinline__ void front_update_t(const double* __restrict__ L, int ld, int p,
-                                               int m, double* t, double* red) {
+                                               int m, double* t) {
   constexpr int NW = NTH / 64;
   const int lane = threadIdx.x & 63;
   // The wave index is uniform: the column addresses stay in scalar registers.
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int CB = (p + 15) >> 4;
-  if (NTH >= 1024 && NW >= 2 * CB) {
-    const int RS = NW / CB;
-    const int cb = w / RS, rs = w - cb * RS;
-    if (cb < CB) {
-      double s[16];
-#pragma unroll
-      for (int u = 0; u < 16; u++) s[u] = 0.0;
-      // Columns past p read column p - 1 (in bounds) and are dropped below.
-      const int ju = min(15, p - 1 - 16 * cb);
-      const double* Lc = L + (size_t)(16 * cb) * ld;
-      for (unsigned i = p + 64 * rs + lane; i < (unsigned)m; i += 64 * RS) {
-        const double ti = t[i];
-        double a[16];
-#pragma unroll
-        for (int u = 0; u < 16; u++) {
-          const double* cu = Lc + __builtin_amdgcn_readfirstlane(min(u, ju) * ld);
-          a[u] = cu[i];
-        }
-#pragma unroll
-        for (int u = 0; u < 16; u++) s[u] += a[u] * ti;
-      }
-      const double tot = wave_reduce16(s, lane);
-      if ((lane & 3) == 0) {
-        const int u = 8 * ((lane >> 5) & 1) + 4 * ((lane >> 4) & 1) + 2 * ((lane >> 3) & 1) +
-                      ((lane >> 2) & 1);
-        red[(rs * CB + cb) * 16 + u] = tot;
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x < p) {
-      const int j = threadIdx.x, cb = j >> 4, u = j & 15;
-      double tot = 0.0;
-      for (int r = 0; r < RS; r++) tot += red[(r * CB + cb) * 16 + u];
-      t[j] -= tot;
-    }
-    __syncthreads();
-    return;
-  }
   for (int jb = 0; jb < p; jb += 16 * NW) {
     if (jb + w >= p) continue;  // no column of this block for this wave
     double s[16];
@@ -2513,7 +2474,7 @@ __global__ __launch_bounds__(NTH) void solve_fwd_kernel(DevPlan P, DevState S,
 }
 
 // Backward: t = x[piv]; t -= L21^T x[rows below]; L11^T solve; x[piv] = t.
-// Dynamic LDS: t (tcap >= m doubles), the packed pivot block, NTH doubles.
+// Dynamic LDS: t (tcap >= m doubles) then the packed pivot block.
 template <int NTH>
 __global__ __launch_bounds__(NTH) void solve_bwd_kernel(DevState S,
                                                        const int* __restrict__ lnodes,
@@ -2539,7 +2500,7 @@ __global__ __launch_bounds__(NTH) void solve_bwd_kernel(DevState S,
       t[j] = xr[v >= 0 ? v : -v - 1];
     }
     __syncthreads();
-    front_update_t<NTH>(L, ld, p, m, t, Ls + PIVOT_TRI);
+    front_update_t<NTH>(L, ld, p, m, t);
     if (wave) {
       if (tid < 64) {
         const double tl = wave_upper_solve(tid < p ? t[tid] : 0.0, tid, p, vars, Ls);
@@ -2934,9 +2895,6 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  150 * 1024));
       HIPCHK(hipFuncSetAttribute((const void*)solve_fwd_kernel<NTS>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 150 * 1024));
-      HIPCHK(hipFuncSetAttribute((const void*)solve_bwd_kernel<NTS>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  150 * 1024));
       attr_done = true;
@@ -3359,12 +3317,8 @@ int LdlNumeric::enqueue_solve(int nrhs, double* x, int ldx, hipStream_t stream,
     const int cnt = LP[l + 1] - LP[l];
     if (cnt == 0) continue;
     tk_begin(stream);
-    if (solve_wide_[l])
-      hipLaunchKernelGGL(solve_bwd_kernel<NTS>, dim3(cnt), dim3(NTS), lds_wide, stream, S_,
-                         LN + LP[l], x, ldx, nrhs, tcap);
-    else
-      hipLaunchKernelGGL(solve_bwd_kernel<NT>, dim3(cnt), dim3(NT), lds, stream, S_,
-                         LN + LP[l], x, ldx, nrhs, tcap);
+    hipLaunchKernelGGL(solve_bwd_kernel<NT>, dim3(cnt), dim3(NT), lds, stream, S_, LN + LP[l], x,
+                       ldx, nrhs, tcap);
     tk_end(stream, TK_SOLVE_BWD);
   }
   if (scaling_)
