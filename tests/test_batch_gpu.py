@@ -81,6 +81,36 @@ def test_union_inertia_and_solves_equal_each_block_alone(scaling):
         assert np.array_equal(xa.cpu().numpy(), xu[offs[k]:offs[k + 1]])
 
 
+def test_union_solves_equal_alone_whatever_the_launch_width():
+    """A level of at most 256 fronts with one of at least 256 rows is
+    forward-solved with 1,024-thread workgroups, other levels with 256.
+    Alone, the KKT below has such levels (fronts of 274, 280 and 311 rows at
+    three levels of 1-3 fronts); in the union with 300 chains (one front at
+    each of 8 levels) the same levels hold over 300 fronts and take the
+    narrow launch.  Its solution must not change by a bit."""
+    import scipy.sparse as sp
+
+    big = M.with_full_diagonal(M.kkt(600, 200, seed=4, density_h=0.005, indefinite_h=True).tocsr())
+    rng = np.random.default_rng(3)
+    chains = []
+    for _ in range(300):
+        n = 500
+        chains.append(sp.diags([np.full(n - 1, -1.0), 4.0 + rng.random(n), np.full(n - 1, -1.0)],
+                               [-1, 0, 1], format="csr"))
+    As = [big] + chains
+    U, mat, ldl, offs = _union(As)
+    ldl.factor()
+    b = rng.normal(size=U.shape[0])
+    xu = torch.tensor(b, device="cuda")
+    ldl.solve(xu)
+    a = _alone(big)
+    xa = torch.tensor(b[:offs[1]], device="cuda")
+    a.solve(xa)
+    assert np.array_equal(xa.cpu().numpy(), xu.cpu().numpy()[:offs[1]])
+    r = big @ xa.cpu().numpy() - b[:offs[1]]
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(b[:offs[1]])
+
+
 def test_masked_factorization_keeps_left_out_factors():
     As = _mats()
     U, mat, ldl, offs = _union(As)
