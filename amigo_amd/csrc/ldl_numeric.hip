@@ -36,6 +36,7 @@ constexpr int KID_BATCH = 64;    // children whose metadata a workgroup holds at
 constexpr int BIG_M = 128;       // static fronts above this use the blocked path
 constexpr int TILE = 64;         // Schur-update tile
 constexpr int DCAP_BIG = 64;     // delayed pivots a blocked front is sized for
+constexpr int NTL = 512;         // level-kernel threads for a level's larger fronts
 constexpr int DPAN_BIG = 16;     // delayed pivots the static panel count covers
 // A front gets blocked-path launches in the static plan when DPLAN_BIG
 // delayed pivots would take it past the LDS limit; one that receives more
@@ -64,6 +65,7 @@ __device__ __forceinline__ VI vi_best(VI a, VI b) {
 }
 
 // Block-wide argmax (largest value, smallest index on ties) -> all threads.
+template <int NTH>
 __device__ VI block_argmax(VI x, VI* red) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -78,7 +80,7 @@ __device__ VI block_argmax(VI x, VI* red) {
   __syncthreads();
   VI r = red[0];
 #pragma unroll
-  for (int w = 1; w < NT / 64; w++) r = vi_best(r, red[w]);
+  for (int w = 1; w < NTH / 64; w++) r = vi_best(r, red[w]);
   return r;
 }
 
@@ -116,18 +118,19 @@ __device__ void symm_swap(double* F, int ld, int m, int* vars, int a, int b) {
   symm_swap_t<NT>(F, ld, m, vars, a, b);
 }
 // The same interchange on an LDS-resident front (32-bit offsets).
+template <int NTH>
 __device__ __forceinline__ void symm_swap_lds(double* F, int ld, int m, int* vars, int a, int b) {
   const int t0 = threadIdx.x;
   __syncthreads();
-  for (int t = t0; t < a; t += NT) swapd(F[a + t * ld], F[b + t * ld]);
+  for (int t = t0; t < a; t += NTH) swapd(F[a + t * ld], F[b + t * ld]);
   if (t0 == 0) {
     swapd(F[a + a * ld], F[b + b * ld]);
     int v = vars[a];
     vars[a] = vars[b];
     vars[b] = v;
   }
-  for (int t = a + 1 + t0; t < b; t += NT) swapd(F[t + a * ld], F[b + t * ld]);
-  for (int t = b + 1 + t0; t < m; t += NT) swapd(F[t + a * ld], F[t + b * ld]);
+  for (int t = a + 1 + t0; t < b; t += NTH) swapd(F[t + a * ld], F[b + t * ld]);
+  for (int t = b + 1 + t0; t < m; t += NTH) swapd(F[t + a * ld], F[t + b * ld]);
   __syncthreads();
 }
 
@@ -216,6 +219,7 @@ __device__ void count_inertia(int step, double d11, double d21, double d22,
 // Unblocked right-looking LDL^T of one front held at F (LDS or global).
 // nfs = fully summed candidates; bk = root front (Bunch-Kaufman, no delay);
 // chol = no pivoting (SolverType::CHOLESKY).  Returns the pivot count.
+template <int NTH>
 __device__ int factor_unblocked(double* F, int ld, int m, int nfs, int* vars,
                                 bool bk, bool chol, double ustab,
                                 double pivot_tol, VI* red, int* cnt) {
@@ -226,15 +230,15 @@ __device__ int factor_unblocked(double* F, int ld, int m, int nfs, int* vars,
     bool zero = false;
     if (!chol) {
       VI a = {-1.0, INT_MAX}, b = {-1.0, INT_MAX};
-      for (int i = k + 1 + tid; i < m; i += NT) {
+      for (int i = k + 1 + tid; i < m; i += NTH) {
         VI c = {fabs(F[i + (size_t)k * ld]), i};
         if (i < nc)
           a = vi_best(a, c);
         else
           b = vi_best(b, c);
       }
-      VI ca = block_argmax(a, red);
-      VI cb = block_argmax(b, red);
+      VI ca = block_argmax<NTH>(a, red);
+      VI cb = block_argmax<NTH>(b, red);
       const double akk = fabs(F[k + (size_t)k * ld]);
       // A column of NaNs selects no row: treat it as having no candidate.
       const bool has_cand = k + 1 < nc && ca.i > k && ca.i < nc;
@@ -247,12 +251,12 @@ __device__ int factor_unblocked(double* F, int ld, int m, int nfs, int* vars,
           kpiv = k;
         } else if (has_cand) {
           VI r = {-1.0, INT_MAX};
-          for (int i = k + tid; i < m; i += NT) {
+          for (int i = k + tid; i < m; i += NTH) {
             if (i == imax) continue;
             double g = i < imax ? F[imax + (size_t)i * ld] : F[i + (size_t)imax * ld];
             r = vi_best(r, VI{fabs(g), i});
           }
-          const double rowmax = block_argmax(r, red).v;
+          const double rowmax = block_argmax<NTH>(r, red).v;
           const double aii = fabs(F[imax + (size_t)imax * ld]);
           if (aii >= pivot_tol && aii >= ustab * rowmax) {
             step = 1;
@@ -286,12 +290,12 @@ __device__ int factor_unblocked(double* F, int ld, int m, int nfs, int* vars,
           kpiv = k;
         } else {
           VI r = {-1.0, INT_MAX};
-          for (int i = k + tid; i < m; i += NT) {
+          for (int i = k + tid; i < m; i += NTH) {
             if (i == imax) continue;
             double g = i < imax ? F[imax + (size_t)i * ld] : F[i + (size_t)imax * ld];
             r = vi_best(r, VI{fabs(g), i});
           }
-          const double rowmax = block_argmax(r, red).v;
+          const double rowmax = block_argmax<NTH>(r, red).v;
           if (akk >= BK_ALPHA * colmax * (colmax / rowmax)) {
             step = 1;
             kpiv = k;
@@ -310,12 +314,12 @@ __device__ int factor_unblocked(double* F, int ld, int m, int nfs, int* vars,
     zero = uni(zero) != 0;
 
     if (step == 0) {
-      if (k < nc - 1) symm_swap(F, ld, m, vars, k, nc - 1);
+      if (k < nc - 1) symm_swap_t<NTH>(F, ld, m, vars, k, nc - 1);
       nc -= 1;
       continue;
     }
     const int ksrc = k + step - 1;
-    if (kpiv != ksrc) symm_swap(F, ld, m, vars, ksrc, kpiv);
+    if (kpiv != ksrc) symm_swap_t<NTH>(F, ld, m, vars, ksrc, kpiv);
 
     if (step == 1) {
       const double d = F[k + (size_t)k * ld];
@@ -323,14 +327,14 @@ __device__ int factor_unblocked(double* F, int ld, int m, int nfs, int* vars,
       if (!zero) {
         const double dinv = 1.0 / d;
         const int tot = w * w;
-        for (int e = tid; e < tot; e += NT) {
+        for (int e = tid; e < tot; e += NTH) {
           const int ii = e % w, jj = e / w;
           if (ii < jj) continue;
           const int i = k + 1 + ii, j = k + 1 + jj;
           F[i + (size_t)j * ld] -= (F[i + (size_t)k * ld] * dinv) * F[j + (size_t)k * ld];
         }
         __syncthreads();
-        for (int i = k + 1 + tid; i < m; i += NT) F[i + (size_t)k * ld] *= dinv;
+        for (int i = k + 1 + tid; i < m; i += NTH) F[i + (size_t)k * ld] *= dinv;
       }
       if (tid == 0) count_inertia(1, d, 0.0, 0.0, zero, cnt);
       __syncthreads();
@@ -343,7 +347,7 @@ __device__ int factor_unblocked(double* F, int ld, int m, int nfs, int* vars,
       const double i00 = c * detinv, i01 = -b * detinv, i11 = a * detinv;
       const int w = m - k - 2;
       const int tot = w * w;
-      for (int e = tid; e < tot; e += NT) {
+      for (int e = tid; e < tot; e += NTH) {
         const int ii = e % w, jj = e / w;
         if (ii < jj) continue;
         const int i = k + 2 + ii, j = k + 2 + jj;
@@ -353,7 +357,7 @@ __device__ int factor_unblocked(double* F, int ld, int m, int nfs, int* vars,
         F[i + (size_t)j * ld] -= l1 * F[j + (size_t)k * ld] + l2 * F[j + (size_t)(k + 1) * ld];
       }
       __syncthreads();
-      for (int i = k + 2 + tid; i < m; i += NT) {
+      for (int i = k + 2 + tid; i < m; i += NTH) {
         const double w1 = F[i + (size_t)k * ld], w2 = F[i + (size_t)(k + 1) * ld];
         F[i + (size_t)k * ld] = w1 * i00 + w2 * i01;
         F[i + (size_t)(k + 1) * ld] = w1 * i01 + w2 * i11;
@@ -537,7 +541,7 @@ __device__ __forceinline__ int wave_argmax_row(double c0, double c1, double mx, 
 // included: the upper triangle of a front is never read (the interchanges
 // and the pivot search touch the lower triangle only) and is not written
 // back.
-template <int ROWS>
+template <int ROWS, int NTH>
 __device__ __forceinline__ int factor_front_lds(double* F, int ld, int m, int nfs, int* vars,
                                                 bool bk, bool chol, double ustab,
                                                 double pivot_tol, int* cnt) {
@@ -545,7 +549,7 @@ __device__ __forceinline__ int factor_front_lds(double* F, int ld, int m, int nf
   // The wave index as a scalar: the update's column indices, bounds and
   // readlane positions derive from it and stay in scalar registers.
   const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-  constexpr int NW = NT / 64;
+  constexpr int NW = NTH / 64;
   const int mlast = m - 1;
   int k = 0, nc = uni(nfs);
   // The pivot block whose columns still hold the unscaled values: its rows
@@ -629,13 +633,13 @@ __device__ __forceinline__ int factor_front_lds(double* F, int ld, int m, int nf
     zero = uni(zero) != 0;
     if (step == 0) {
       // symm_swap_lds starts with a barrier: every wave has read what it moves.
-      if (k < nc - 1) symm_swap_lds(F, ld, m, vars, k, nc - 1);
+      if (k < nc - 1) symm_swap_lds<NTH>(F, ld, m, vars, k, nc - 1);
       nc -= 1;
       continue;
     }
     const int ksrc = k + step - 1;
     if (kpiv != ksrc) {
-      symm_swap_lds(F, ld, m, vars, ksrc, kpiv);
+      symm_swap_lds<NTH>(F, ld, m, vars, ksrc, kpiv);
       v0 = F[c0 + k * ld];
       if (ROWS == 2) v1 = F[c1 + k * ld];
     } else if (read_imax) {
@@ -795,15 +799,16 @@ __device__ __forceinline__ int fail_flag(const DevState& S) {
 
 // One workgroup per front of the level: allocate, assemble, and (unless the
 // front is routed to the blocked path) factor.
-__global__ __launch_bounds__(NT) void level_kernel(
+template <int NTH>
+__global__ __launch_bounds__(NTH) void level_kernel(
     DevPlan P, DevState S, const double* __restrict__ vals,
     const double* __restrict__ shift, const int* __restrict__ lnodes,
     const int* __restrict__ big_mask, int lds_doubles, double ustab,
     double pivot_tol, int chol, int dcap_big, const double* __restrict__ sc,
     const int* __restrict__ node_cap) {
   extern __shared__ double smem[];  // F (lds_doubles) then the cached column
-  __shared__ VI red[2 * NT / 64];
-  __shared__ VI cred[2 * NT / 64];
+  __shared__ VI red[2 * NTH / 64];
+  __shared__ VI cred[2 * NTH / 64];
   __shared__ int sh[4];
   __shared__ int cnt[4];
   __shared__ KidMeta km[KID_BATCH];
@@ -821,7 +826,7 @@ __global__ __launch_bounds__(NT) void level_kernel(
   }
   __syncthreads();
   // Delayed pivots of the children, summed in parallel (integer sum).
-  for (int q = kb + tid; q < ke; q += NT) {
+  for (int q = kb + tid; q < ke; q += NTH) {
     const int nd = S.ndel[P.kids[q]];
     if (nd) atomicAdd(&sh[2], nd);
   }
@@ -879,8 +884,8 @@ __global__ __launch_bounds__(NT) void level_kernel(
 
   // Front variables: pivots of the supernode, delayed pivots of the
   // children (in child order), static contribution rows.
-  for (int j = tid; j < ns; j += NT) vars[j] = P.nvars[P.nptr[node] + j];
-  for (int j = tid; j < cs; j += NT) vars[ns + D + j] = P.crows[P.cptr[node] + j];
+  for (int j = tid; j < ns; j += NTH) vars[j] = P.nvars[P.nptr[node] + j];
+  for (int j = tid; j < cs; j += NTH) vars[ns + D + j] = P.crows[P.cptr[node] + j];
   // Children's metadata, KID_BATCH children at a time (one batch for
   // almost every front, kept for the extend-add below).
   auto load_kids = [&](int q0, int nb) {
@@ -904,7 +909,7 @@ __global__ __launch_bounds__(NT) void level_kernel(
       for (int qq = 0; qq < nb; qq++) {
         const int nd = km[qq].dc;
         const int* cv = S.ipool + km[qq].voff + km[qq].pc;
-        for (int t = tid; t < nd; t += NT) vars[off + t] = cv[t];
+        for (int t = tid; t < nd; t += NTH) vars[off + t] = cv[t];
         off += nd;
       }
       __syncthreads();  // km is reloaded next
@@ -927,11 +932,11 @@ __global__ __launch_bounds__(NT) void level_kernel(
   // global memory.
   auto assemble = [&](double* F) __attribute__((always_inline)) {
     const size_t mm = (size_t)m * m;
-    for (size_t e = tid; e < mm; e += NT) F[e] = 0.0;
+    for (size_t e = tid; e < mm; e += NTH) F[e] = 0.0;
     __syncthreads();
 
     // Original entries of the supernode's columns.
-    for (int t = P.aptr[node] + tid; t < P.aptr[node + 1]; t += NT) {
+    for (int t = P.aptr[node] + tid; t < P.aptr[node + 1]; t += NTH) {
       int r = P.arow[t];
       r = r < ns ? r : r + D;
       const int c = P.acol[t];
@@ -944,7 +949,7 @@ __global__ __launch_bounds__(NT) void level_kernel(
     }
     __syncthreads();
     if (shift != nullptr) {
-      for (int j = tid; j < ns; j += NT) {
+      for (int j = tid; j < ns; j += NTH) {
         const int v = vars[j];
         F[j + (size_t)j * ld] += sc ? shift[v] * sc[v] * sc[v] : shift[v];
       }
@@ -965,7 +970,7 @@ __global__ __launch_bounds__(NT) void level_kernel(
           const int pc = km[qq].pc, dc = km[qq].dc, mc = km[qq].mc;
           const int w = mc - pc;
           const int* cm = P.cmap + km[qq].cptr - dc;  // cm[ii] for ii >= dc
-          for (int ii = tid; ii < w; ii += NT) {
+          for (int ii = tid; ii < w; ii += NTH) {
             int pos;
             if (ii < dc) {
               pos = doff + ii;
@@ -978,16 +983,16 @@ __global__ __launch_bounds__(NT) void level_kernel(
           __syncthreads();
           const double* Cf = S.rpool + km[qq].foff + pc + (size_t)pc * mc;
           const int tot = w * w;
-          for (int e0 = tid; e0 < tot; e0 += 16 * NT) {
+          for (int e0 = tid; e0 < tot; e0 += 16 * NTH) {
             double a[16];
   #pragma unroll
             for (int u = 0; u < 16; u++) {
-              const int e = e0 + u * NT, jj = e / w, ii = e - jj * w;
+              const int e = e0 + u * NTH, jj = e / w, ii = e - jj * w;
               a[u] = (e < tot && ii >= jj) ? Cf[ii + (size_t)jj * mc] : 0.0;
             }
   #pragma unroll
             for (int u = 0; u < 16; u++) {
-              const int e = e0 + u * NT, jj = e / w, ii = e - jj * w;
+              const int e = e0 + u * NTH, jj = e / w, ii = e - jj * w;
               if (e < tot && ii >= jj) {
                 const int pi = rowpos[ii], pj = rowpos[jj];
                 if (pi >= pj)
@@ -1013,18 +1018,18 @@ __global__ __launch_bounds__(NT) void level_kernel(
   int np;
   if (in_lds) {
     if (m <= 64)
-      np = factor_front_lds<1>(smem, m, m, ns + D, vars, cs == 0, chol != 0, ustab, pivot_tol,
+      np = factor_front_lds<1, NTH>(smem, m, m, ns + D, vars, cs == 0, chol != 0, ustab, pivot_tol,
                                cnt);
     else
-      np = factor_front_lds<2>(smem, m, m, ns + D, vars, cs == 0, chol != 0, ustab, pivot_tol,
+      np = factor_front_lds<2, NTH>(smem, m, m, ns + D, vars, cs == 0, chol != 0, ustab, pivot_tol,
                                cnt);
     // The lower triangle (nothing reads a front above its diagonal).
-    for (int e = tid; e < m * m; e += NT) {
+    for (int e = tid; e < m * m; e += NTH) {
       const int j = e / m, i = e - j * m;
       if (i >= j) G[e] = smem[e];
     }
   } else {
-    np = factor_unblocked(G, ld, m, ns + D, vars, cs == 0, chol != 0, ustab,
+    np = factor_unblocked<NTH>(G, ld, m, ns + D, vars, cs == 0, chol != 0, ustab,
                           pivot_tol, red, cnt);
   }
   if (tid == 0) {
@@ -2873,7 +2878,10 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   {
     static bool attr_done = false;
     if (!attr_done) {
-      HIPCHK(hipFuncSetAttribute((const void*)level_kernel,
+      HIPCHK(hipFuncSetAttribute((const void*)level_kernel<NT>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (LDS_MAX_M * LDS_MAX_M + 2 * LDS_MAX_M) * (int)sizeof(double)));
+      HIPCHK(hipFuncSetAttribute((const void*)level_kernel<NTL>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (LDS_MAX_M * LDS_MAX_M + 2 * LDS_MAX_M) * (int)sizeof(double)));
       HIPCHK(hipFuncSetAttribute((const void*)panel_kernel<true, NTP>,
@@ -2988,7 +2996,18 @@ int LdlNumeric::enqueue_factor(const double* vals, const double* shift,
     for (const auto& pt : parts) {
       if (pt[1] == 0) continue;
       const int lds = pt[2];
-      hipLaunchKernelGGL(level_kernel, dim3(pt[1]), dim3(NT),
+      // The larger fronts (one workgroup per CU by their LDS) take NTL
+      // threads: more waves share each pivot step's update.
+      if (&pt == &parts[0])
+        hipLaunchKernelGGL(level_kernel<NT>, dim3(pt[1]), dim3(NT),
+                         size_t(lds + 2 * LDS_MAX_M) * sizeof(double), stream, P_, S_, vals,
+                         shift, LN + pt[0], d_big_mask_, lds,
+                         prm.ustab, prm.pivot_tol, cholesky ? 1 : 0,
+                         sync_mode ? -1 : DCAP_BIG,
+                         (const double*)(scaling_ ? scale_ : nullptr),
+                         (const int*)(nbatch_ ? d_node_cap_ : nullptr));
+      else
+        hipLaunchKernelGGL(level_kernel<NTL>, dim3(pt[1]), dim3(NTL),
                          size_t(lds + 2 * LDS_MAX_M) * sizeof(double), stream, P_, S_, vals,
                          shift, LN + pt[0], d_big_mask_, lds,
                          prm.ustab, prm.pivot_tol, cholesky ? 1 : 0,
