@@ -216,12 +216,10 @@ class LdlNumeric {
   std::vector<int> h_m_;            // front sizes read back in sync mode
   const int* d_big_nodes_ = nullptr;
   const int* d_big_root_ = nullptr;
-  const int* d_big_ns_ = nullptr;
   const int64_t* d_big_woff_ = nullptr;
   int64_t* d_big_woff_sync_ = nullptr;
   int* d_sync_nodes_ = nullptr;
   int* d_sync_root_ = nullptr;
-  int* d_sync_ns_ = nullptr;
   int delayed_ = 0, max_front_seen_ = 0;
   std::vector<int> level_lds_;      // LDS doubles of a level's larger fronts
   std::vector<int> level_split_;    // first larger front of each level
@@ -261,14 +259,13 @@ class LdlNumeric {
   // LDS-bucket splits, and the blocked-path plan of those fronts.
   bool mask_all_ = true;
   std::vector<int> m_level_ptr_, m_level_split_, m_level_nodes_, m_big_ptr_;
-  std::vector<int> m_big_nodes_, m_big_root_, m_big_ns_;
+  std::vector<int> m_big_nodes_, m_big_root_;
   std::vector<int64_t> m_big_woff_;
   int* d_m_level_nodes_ = nullptr;
   int* d_m_big_nodes_ = nullptr;
   int* d_m_big_root_ = nullptr;
-  int* d_m_big_ns_ = nullptr;
   int64_t* d_m_big_woff_ = nullptr;
-  std::vector<int> h_big_root_, h_big_ns_;
+  std::vector<int> h_big_root_;
   std::vector<int64_t> h_big_woff_;
   std::vector<int> h_nptr_, h_nvars_;
   int* d_row_blk_ = nullptr;        // [n] block of every row (batches)

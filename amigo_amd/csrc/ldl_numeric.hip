@@ -825,9 +825,14 @@ __global__ __launch_bounds__(NTH) void level_kernel(
     cnt[0] = cnt[1] = cnt[2] = cnt[3] = 0;
   }
   __syncthreads();
-  // Delayed pivots of the children, summed in parallel (integer sum).
+  // Delayed pivots of the children, summed in parallel (integer sum).  A
+  // blocked child that the static panel launches did not finish has no
+  // counts: the factorization is redone with per-level sizes.
   for (int q = kb + tid; q < ke; q += NTH) {
-    const int nd = S.ndel[P.kids[q]];
+    const int c = P.kids[q];
+    const int* pc = S.pstate + PS * c;
+    if (pc[4] == 1 && pc[0] < pc[1]) atomicExch(&S.flags[0], -7);
+    const int nd = S.ndel[c];
     if (nd) atomicAdd(&sh[2], nd);
   }
   __syncthreads();
@@ -923,6 +928,7 @@ __global__ __launch_bounds__(NTH) void level_kernel(
       S.pstate[PS * node + 2] = 0;
       S.pstate[PS * node + 3] = 0;
       S.pstate[PS * node + 4] = 1;
+      S.pstate[PS * node + 5] = ns + D;  // fully summed rows, for the delay count
       for (int q = 0; q < 4; q++) S.ninert[4 * node + q] = 0;
     }
     return;
@@ -1804,6 +1810,11 @@ __global__ __launch_bounds__(REG ? NTH : NTP) void panel_kernel(DevState S,
     ps[1] = nc;
     ps[2] = kstart;
     ps[3] = kw;
+    if (k >= nc) {
+      // Finished: pivot and delay counts for the parent and the solves.
+      S.npiv[node] = nc;
+      S.ndel[node] = ps[5] - nc;
+    }
     for (int q = 0; q < 4; q++)
       if (cnt[q]) atomicAdd(&S.ninert[4 * node + q], cnt[q]);
   }
@@ -1923,26 +1934,6 @@ __global__ __launch_bounds__(NT) void trailing_mfma_kernel(DevState S,
       }
 }
 
-// Close the blocked fronts of a level: pivot/delay counts, and a check that
-// the statically sized panel sequence finished the front (otherwise the
-// factorization is redone level-synchronously).
-__global__ void finalize_big_kernel(DevState S, const int* __restrict__ bnodes,
-                                    const int* __restrict__ bns, int count) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= count || fail_flag(S) != 0) return;
-  const int node = bnodes[t];
-  if (!node_on(S, node)) return;
-  if (S.pstate[PS * node + 4] == 0) return;  // factored in a workgroup
-  const int k = S.pstate[PS * node + 0], nc = S.pstate[PS * node + 1];
-  if (k < nc) {
-    atomicExch(&S.flags[0], -7);
-    return;
-  }
-  const int nfs = bns[t] + S.D[node];
-  S.npiv[node] = nc;
-  S.ndel[node] = nfs - nc;
-}
-
 // flags[1..4] = inertia counts (+, -, 0, non-finite), flags[5] = delayed
 // pivots, flags[6] = largest front, over the fronts this factorization
 // took part in, from the per-node counts (integer sums: exact in any order).
@@ -1952,6 +1943,8 @@ __global__ __launch_bounds__(NT) void factor_stats_kernel(DevState S, int K) {
   int c[6] = {0, 0, 0, 0, 0, 0};
   for (int node = blockIdx.x * NT + threadIdx.x; node < K; node += gridDim.x * NT) {
     if (!node_on(S, node)) continue;
+    const int* ps = S.pstate + PS * node;
+    if (ps[4] == 1 && ps[0] < ps[1]) atomicExch(&S.flags[0], -7);  // a root left unfinished
     for (int q = 0; q < 4; q++) c[q] += S.ninert[4 * node + q];
     c[4] += S.ndel[node];
     c[5] = max(c[5], S.m[node]);
@@ -2077,6 +2070,17 @@ __global__ __launch_bounds__(NT) void fpos_kernel(DevPlan P, DevState S, int K) 
     int* fpos = S.fpos + S.voff[node];
     const int* sv = P.nvars + P.nptr[node];
     const int kb = P.kptr[node], ke = P.kptr[node + 1];
+    // A blocked child that its panel launches did not finish (the root
+    // levels have no parent level kernel to see it): redo, map nothing.
+    bool kids_done = true;
+    for (int t = kb; t < ke; t++) {
+      const int* pc = S.pstate + PS * P.kids[t];
+      if (pc[4] == 1 && pc[0] < pc[1]) kids_done = false;
+    }
+    if (!kids_done) {
+      if (threadIdx.x == 0) atomicExch(&S.flags[0], -7);
+      continue;
+    }
     for (int j = threadIdx.x; j < nfs; j += NT) {
       int v = vars[j];
       v = v >= 0 ? v : -v - 1;
@@ -2590,7 +2594,7 @@ LdlNumeric::~LdlNumeric() {
   if (h_binert_pin_) (void)hipHostFree(h_binert_pin_);
   if (h_stage_pin_) (void)hipHostFree(h_stage_pin_);
   for (void* q : {(void*)d_m_level_nodes_, (void*)d_m_big_nodes_, (void*)d_m_big_root_,
-                  (void*)d_m_big_ns_, (void*)d_m_big_woff_, (void*)d_row_blk_,
+                  (void*)d_m_big_woff_, (void*)d_row_blk_,
                   (void*)d_node_cap_})
     if (q) (void)hipFree(q);
 }
@@ -2768,7 +2772,7 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   big_wide_.assign(nlev, 0);
   big_mmax_.assign(nlev, 0);
   h_kptr_ = plan.children_ptr;
-  std::vector<int> big_root, big_ns;
+  std::vector<int> big_root;
   std::vector<int64_t> big_woff;
   int64_t wmax = 0;
   big_nth_.assign(nlev, NTP);
@@ -2782,7 +2786,6 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
       if (big[k]) {
         big_nodes_.push_back(k);
         big_root.push_back(cs[k] == 0 ? 1 : 0);
-        big_ns.push_back(ns[k]);
         const int mcap = ns[k] + cs[k] + DCAP_BIG;
         big_woff.push_back(wtot);
         wtot += (int64_t)mcap * NBW;
@@ -2841,18 +2844,14 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   {
     const int* t1 = nullptr;
     const int* t2 = nullptr;
-    const int* t3 = nullptr;
     const int64_t* t4 = nullptr;
     h_big_root_ = big_root;
-    h_big_ns_ = big_ns;
     h_big_woff_ = big_woff;
     if (upload(allocs_, big_nodes_, &t1, err)) return -2;
     if (upload(allocs_, big_root, &t2, err)) return -2;
-    if (upload(allocs_, big_ns, &t3, err)) return -2;
     if (upload(allocs_, big_woff, &t4, err)) return -2;
     d_big_nodes_ = t1;
     d_big_root_ = t2;
-    d_big_ns_ = t3;
     d_big_woff_ = t4;
     if (dalloc(allocs_, std::max<size_t>(1, (size_t)K), &d_big_woff_sync_, err)) return -2;
     if (wmax > 0) {
@@ -2870,7 +2869,6 @@ int LdlNumeric::setup(const SymbolicPlan& plan, const int* rowp,
   if (dalloc(allocs_, PS * (size_t)K, &S_.pstate, err)) return -2;
   if (dalloc(allocs_, K, &d_sync_nodes_, err)) return -2;
   if (dalloc(allocs_, K, &d_sync_root_, err)) return -2;
-  if (dalloc(allocs_, K, &d_sync_ns_, err)) return -2;
   if (dalloc(allocs_, 2, &S_.tops, err)) return -2;
   if (dalloc(allocs_, 8, &S_.flags, err)) return -2;
   if (dalloc(allocs_, 4 * (size_t)K, &S_.ninert, err)) return -2;
@@ -3022,7 +3020,6 @@ int LdlNumeric::enqueue_factor(const double* vals, const double* shift,
     int npanels = big_npanels_[l], tiles = big_tiles_[l];
     const int* bnodes = (ml ? d_m_big_nodes_ : d_big_nodes_) + b0;
     const int* broot = (ml ? d_m_big_root_ : d_big_root_) + b0;
-    const int* bns = (ml ? d_m_big_ns_ : d_big_ns_) + b0;
     const int64_t* woff = (ml ? d_m_big_woff_ : d_big_woff_) + b0;
     int nlaunch = nb;
     if (sync_mode) {
@@ -3039,7 +3036,7 @@ int LdlNumeric::enqueue_factor(const double* vals, const double* shift,
       HIPCHK(hipMemcpyAsync(hm.data(), S_.m, K_ * sizeof(int), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipMemcpyAsync(hD.data(), S_.D, K_ * sizeof(int), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
-      std::vector<int> ln, lr, lns;
+      std::vector<int> ln, lr;
       std::vector<int64_t> wo;
       int64_t tot = 0;
       npanels = 0;
@@ -3050,7 +3047,6 @@ int LdlNumeric::enqueue_factor(const double* vals, const double* shift,
         if (hm[node] <= LDS_MAX_M) continue;
         ln.push_back(node);
         lr.push_back(h_cs_[node] == 0 ? 1 : 0);
-        lns.push_back(h_ns_[node]);
         wo.push_back(tot);
         tot += (int64_t)hm[node] * NBW;
         npanels = std::max(npanels, (h_ns_[node] + hD[node] + NBR - 1) / NBR);
@@ -3071,12 +3067,9 @@ int LdlNumeric::enqueue_factor(const double* vals, const double* shift,
                             hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpyAsync(d_sync_root_, lr.data(), nlaunch * sizeof(int),
                             hipMemcpyHostToDevice, stream));
-      HIPCHK(hipMemcpyAsync(d_sync_ns_, lns.data(), nlaunch * sizeof(int),
-                            hipMemcpyHostToDevice, stream));
       HIPCHK(hipStreamSynchronize(stream));  // the host lists are loop locals
       bnodes = d_sync_nodes_;
       broot = d_sync_root_;
-      bns = d_sync_ns_;
       woff = d_big_woff_sync_;
     }
     {
@@ -3132,8 +3125,6 @@ int LdlNumeric::enqueue_factor(const double* vals, const double* shift,
                            dim3(NT), 0, stream, S_, bnodes, (const double*)W_, woff);
       tk_end(stream, TK_TRAILING);
     }
-    hipLaunchKernelGGL(finalize_big_kernel, dim3((nlaunch + 63) / 64), dim3(64), 0,
-                       stream, S_, bnodes, bns, nlaunch);
     HIPCHK(hipGetLastError());
   }
   hipLaunchKernelGGL(fpos_kernel, dim3(std::min(K_, 4096)), dim3(NT), 0, stream, P_, S_, K_);
@@ -3504,13 +3495,11 @@ int LdlNumeric::set_batch(int nblocks, const int* node_seg, std::string& err) {
   fr(d_m_level_nodes_);
   fr(d_m_big_nodes_);
   fr(d_m_big_root_);
-  fr(d_m_big_ns_);
   fr(d_m_big_woff_);
   HIPCHK(hipMalloc(&d_m_level_nodes_, sizeof(int) * std::max(K_, 1)));
   const size_t nbig = std::max<size_t>(1, big_nodes_.size());
   HIPCHK(hipMalloc(&d_m_big_nodes_, sizeof(int) * nbig));
   HIPCHK(hipMalloc(&d_m_big_root_, sizeof(int) * nbig));
-  HIPCHK(hipMalloc(&d_m_big_ns_, sizeof(int) * nbig));
   HIPCHK(hipMalloc(&d_m_big_woff_, sizeof(int64_t) * nbig));
   HIPCHK(hipMalloc(&d_node_blk_, sizeof(int) * std::max(K_, 1)));
   HIPCHK(hipMemcpy(d_node_blk_, h_node_blk_.data(), sizeof(int) * K_, hipMemcpyHostToDevice));
@@ -3601,7 +3590,6 @@ int LdlNumeric::set_batch_mask(const int* mask, hipStream_t stream, std::string&
     m_big_ptr_.assign(nlev + 1, 0);
     m_big_nodes_.clear();
     m_big_root_.clear();
-    m_big_ns_.clear();
     m_big_woff_.clear();
     auto on_node = [&](int node) { return h_blk_active_[h_node_blk_[node]] != 0; };
     for (int l = 0; l < nlev; l++) {
@@ -3615,7 +3603,6 @@ int LdlNumeric::set_batch_mask(const int* mask, hipStream_t stream, std::string&
         if (!on_node(big_nodes_[t])) continue;
         m_big_nodes_.push_back(big_nodes_[t]);
         m_big_root_.push_back(h_big_root_[t]);
-        m_big_ns_.push_back(h_big_ns_[t]);
         m_big_woff_.push_back(h_big_woff_[t]);
       }
       m_big_ptr_[l + 1] = int(m_big_nodes_.size());
@@ -3636,7 +3623,6 @@ int LdlNumeric::set_batch_mask(const int* mask, hipStream_t stream, std::string&
       const size_t nb = m_big_nodes_.size();
       if (up(d_m_big_nodes_, m_big_nodes_.data(), sizeof(int) * nb) ||
           up(d_m_big_root_, m_big_root_.data(), sizeof(int) * nb) ||
-          up(d_m_big_ns_, m_big_ns_.data(), sizeof(int) * nb) ||
           up(d_m_big_woff_, m_big_woff_.data(), sizeof(int64_t) * nb))
         return -2;
     }

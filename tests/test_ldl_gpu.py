@@ -212,6 +212,64 @@ def test_delays_with_wide_fronts_match_oracle():
     assert np.allclose(x.cpu().numpy(), O.solve(b), rtol=1e-9, atol=1e-9)
 
 
+def _delays_beyond_static_panels_matrix(nleaf=40, ns=64, tail=60, nroot=0, seed=5):
+    """NATURAL order: nleaf one-variable leaves with a zero diagonal, each
+    coupled to all ns variables of a dense block (ns = 64 columns, so no
+    leaf is merged into it), then a tail of rows coupled to the whole block,
+    then nroot rows coupled to the whole tail.  Every leaf delays its pivot,
+    so the front that takes them holds 40 more candidates than its static
+    panel plan (its own pivots plus 16).  With nroot = 0 the block and the
+    tail form one root front; with nroot > 0 the block's front (contribution
+    rows: the tail) has the tail and root rows' front as its parent."""
+    from scipy.sparse import csr_matrix
+
+    rng = np.random.default_rng(seed)
+    n = nleaf + ns + tail + nroot
+    A = np.zeros((n, n))
+    b0, t0, r0 = nleaf, nleaf + ns, nleaf + ns + tail
+    B = rng.normal(size=(ns, ns))
+    A[b0:t0, b0:t0] = B + B.T + np.diag(np.full(ns, 2.0 * ns))
+    for i in range(nleaf):
+        A[i, b0:t0] = A[b0:t0, i] = rng.normal(size=ns)
+    for t in range(tail):
+        i = t0 + t
+        A[i, i] = 4.0 if t % 2 else -3.0
+        A[i, b0:t0] = A[b0:t0, i] = 0.1 * rng.normal(size=ns)
+        if nroot:
+            A[i, r0:] = A[r0:, i] = 0.1 * rng.normal(size=nroot)
+    for q in range(nroot):
+        A[r0 + q, r0 + q] = 5.0
+    M_ = csr_matrix(A + np.diag(np.full(n, 1e-300)))
+    M_.data[np.abs(M_.data) <= 1e-299] = 0.0
+    M_.sort_indices()
+    return M_
+
+
+@pytest.mark.parametrize("nroot", [0, 8])
+def test_delays_beyond_the_static_panel_plan(nroot):
+    """A blocked front with more delayed pivots than its planned panel
+    launches hold is found unfinished (as a root front, and as a front
+    whose parent's level then sees it) and the factorization is redone
+    with per-level sizes: inertia equal to the eigenvalue inertia, pivot
+    counts showing every leaf delayed, residual of a solve <= 1e-10."""
+    from amigo_amd.sparse import CSRMat, SparseLDL
+
+    A = _delays_beyond_static_panels_matrix(nroot=nroot)
+    ldl = SparseLDL(CSRMat.from_scipy(A), order="NATURAL")
+    for _ in range(2):  # the second factorization reuses the handle's plan
+        ldl.factor()
+        Ad = A.toarray()
+        ev = np.linalg.eigvalsh(Ad)
+        assert ldl.get_inertia() == (int(np.sum(ev >= 0)), int(np.sum(ev < 0)))
+        npiv, ndel = ldl.pivot_counts()
+        assert ndel.sum() >= 40
+        b = np.random.default_rng(2).normal(size=A.shape[0])
+        x = torch.tensor(b, device="cuda")
+        ldl.solve(x)
+        r = np.linalg.norm(Ad @ x.cpu().numpy() - b) / np.linalg.norm(b)
+        assert r < 1e-10
+
+
 def _tall_front_matrix(nb=1100, tail=48, seed=11):
     """A dense symmetric indefinite block of nb rows (every tenth diagonal
     entry zero, so 2x2 pivots occur) followed by a tail of
